@@ -1,0 +1,222 @@
+/*
+ * letkf_gpu.h -- C ABI of the MI355X local-analysis library.
+ *
+ * One handle holds one analysis on one GPU: the background ensemble of the
+ * local gridpoints, the observations, the localizer and the solver
+ * parameters.  letkf_gpu_run() performs the work of the reference's
+ * letkf_solver_run() (src/letkf_solver.F90:150-389): local-observation
+ * search, Gaspari-Cohn localization, the LETKF ensemble transform
+ * (src/letkf_core.F90:69-146), the application of the transform to every
+ * state slab and the post-processing (recentering, increment clipping,
+ * RTPS / RTPP / multiplicative inflation, member bounds, spread).
+ *
+ * Conventions
+ *  - Every function returns 0 (LETKF_OK) or a LETKF_E* status;
+ *    letkf_gpu_last_error() returns the message of the last failure.
+ *  - All arrays are Fortran column-major with the same index order as the
+ *    reference's module arrays; sizes are int64_t.
+ *  - The caller owns every host buffer.  The set_* calls copy host data to
+ *    device memory owned by the handle; the get_* calls copy results back.
+ *    Nothing the caller passes is kept or freed.
+ *  - Calls are synchronous.  A handle owns one HIP stream (default
+ *    priority) and is not thread safe; use one handle per host thread.
+ *  - Fortran LOGICAL arrays are passed as int32 (non-zero = .TRUE.).
+ *  - Gridpoint, slab and observation indices are 0-based at this ABI.
+ */
+#ifndef LETKF_GPU_H
+#define LETKF_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define LETKF_OK            0
+#define LETKF_EINVAL        1  /* invalid argument or parameter value       */
+#define LETKF_EUNSUPPORTED  2  /* localizer class / option not on the GPU   */
+#define LETKF_ENOMEM        3  /* device allocation failed                  */
+#define LETKF_EHIP          4  /* HIP runtime error                         */
+#define LETKF_ECAPACITY     5  /* a gridpoint's search set exceeds max_obs  */
+#define LETKF_ESTATE        6  /* call order violated (e.g. run before set) */
+
+/* ---- arithmetic of the local analysis ---------------------------------- */
+#define LETKF_PREC_FP64     0  /* everything in FP64                        */
+#define LETKF_PREC_FP32     1  /* products in FP32, eigensolve in FP64      */
+#define LETKF_PREC_BF16X3   2  /* products split-BF16 (3 MFMA terms), FP32
+                                  accumulation; eigensolve in FP64          */
+#define LETKF_PREC_BF16     3  /* products plain BF16, FP32 accumulation     */
+
+/* ---- execution path ---------------------------------------------------- */
+#define LETKF_PATH_PLAIN    0  /* one gridpoint per workgroup, unbucketed   */
+#define LETKF_PATH_BATCHED  1  /* gridpoint tiles bucketed by local obs count */
+
+/* ---- localizer classes (YAML localization.class) ------------------------ */
+#define LETKF_LOC_NOVRT     1  /* subclass.letkf_localizer/letkf_loc_novrt.F90 */
+#define LETKF_LOC_OCEAN     2  /* subclass.letkf_localizer/letkf_loc_ocean.F90 */
+#define LETKF_LOC_VGAUSS    3  /* horizontal GC x vertical Gaussian, one group
+                                  per level (new class, no reference file)   */
+
+#define LETKF_MAX_INTERP    32
+#define LETKF_MAX_IDLIST    64
+
+/* Problem sizes, fixed for the life of a handle.
+ * Replaces: mpi.ens_size (src/letkf_mpi.F90:134), grid_ns and ij_count
+ * (src/letkf_state.F90:182-192), max_obs (src/letkf_solver.F90:165). */
+typedef struct letkf_gpu_dims {
+  int64_t ens_size;   /* k                                                  */
+  int64_t grid_ns;    /* slabs per gridpoint (sum of levels over variables) */
+  int64_t ij_count;   /* gridpoints owned by this handle                    */
+  int64_t max_obs;    /* search-set capacity per gridpoint; 0 -> 100000     */
+  int32_t precision;  /* LETKF_PREC_*                                       */
+  int32_t path;       /* LETKF_PATH_*                                       */
+} letkf_gpu_dims;
+
+/* One `linearinterp_lat` table exactly as written in the YAML `value:` list
+ * (src/letkf_loc.F90:251-294).  The library adds the implicit 0 and 90
+ * degree end points the same way the reference does. */
+typedef struct letkf_linearinterp_lat {
+  int32_t npoints;
+  float   lat[LETKF_MAX_INTERP];     /* degrees, ascending */
+  float   radius[LETKF_MAX_INTERP];  /* metres             */
+} letkf_linearinterp_lat;
+
+/* Localizer descriptor.  Replaces the letkf_localizer object selected by
+ * localization.class (src/letkf_loc.F90:54-110,164-210). */
+typedef struct letkf_loc_desc {
+  int32_t cls;                        /* LETKF_LOC_*                         */
+  /* LOC_NOVRT and LOC_VGAUSS: hzloc (letkf_loc_novrt.F90:98-105)          */
+  letkf_linearinterp_lat hzloc;
+  /* LOC_OCEAN (letkf_loc_ocean.F90:120-232)                                */
+  letkf_linearinterp_lat hzloc_prof;
+  letkf_linearinterp_lat hzloc_sat;
+  float   tloc_prof;                  /* <= 0 disables (default -1)          */
+  float   tloc_sat;
+  int32_t vtloc_surf;                 /* 0 = none, 1 = bkg_t                 */
+  float   bkg_t_delta;
+  int32_t bkg_t_var_idx;              /* first slab of bkg_t_var (0-based)   */
+  int32_t bkg_t_var_len;              /* levels of bkg_t_var                 */
+  int32_t n_sat_obs;                  /* obsid list, 0 = empty list          */
+  int32_t sat_obs[LETKF_MAX_IDLIST];
+  int32_t n_sat_plats;                /* platid list, 0 = empty list         */
+  int32_t sat_plats[LETKF_MAX_IDLIST];
+  /* LOC_VGAUSS                                                             */
+  float   vt_scale;                   /* Gaussian scale in zdim units        */
+  int32_t nlev;                       /* number of levels                    */
+  const float* lev_depth;             /* nlev level depths (copied)          */
+} letkf_loc_desc;
+
+/* Per-gridpoint diagnostics (the fields of diag.solver.nc,
+ * src/letkf_solver.F90:59-62,212,220,265-268).  Any pointer may be NULL.
+ * lg_* arrays are (ij_count, maxgroups) column-major like the reference. */
+typedef struct letkf_gpu_diag {
+  float*    maxhz;        /* ij_count                                  */
+  float*    obscount;     /* ij_count: size of the search set          */
+  float*    lg_obscount;  /* kept observations per group               */
+  float*    lg_obsloc;    /* sum of localization weights per group     */
+  uint64_t* lg_selhash;   /* order-free hash of the kept index set     */
+} letkf_gpu_diag;
+
+/* Phase timings of the last letkf_gpu_run (HIP events on the handle's
+ * stream), in milliseconds. */
+typedef struct letkf_gpu_timing {
+  double search_ms;    /* local-obs search + localization          */
+  double product_ms;   /* Yb^T R^-1 Yb and Yb^T R^-1 d              */
+  double eigen_ms;     /* eigensolve, transform, application        */
+  double finalize_ms;  /* recenter / clip / inflation / spread      */
+  double total_ms;
+  int64_t n_problems;  /* (gridpoint, group) solves with m > 0       */
+  int64_t n_pairs;     /* sum of kept observations over all solves   */
+  int64_t n_cand;      /* candidate (gridpoint, obs) pairs tested     */
+} letkf_gpu_timing;
+
+typedef struct letkf_gpu_t letkf_gpu_t;
+
+/* Library version string. */
+const char* letkf_gpu_version(void);
+
+/* Create a handle on the current HIP device.
+ * Replaces letkf_core_init (src/letkf_core.F90:56-61) and the buffer
+ * set-up of letkf_solver_init (src/letkf_solver.F90:127-140). */
+int letkf_gpu_create(const letkf_gpu_dims* dims, letkf_gpu_t** handle);
+
+/* Release every device resource of the handle (letkf_solver_final's
+ * deallocation, src/letkf_solver.F90:397-527).  NULL is accepted. */
+int letkf_gpu_destroy(letkf_gpu_t* handle);
+
+/* Message of the last failed call on this handle ("" if none).  With a
+ * NULL handle, the message of the last failed letkf_gpu_create. */
+const char* letkf_gpu_last_error(const letkf_gpu_t* handle);
+
+/* Background ensemble of the local gridpoints (src/letkf_state.F90:182-192):
+ *   state_ij (k, ns, ij_count)  background perturbations (mean removed)
+ *   mean_ij  (ns, ij_count)     background mean
+ *   sprd_ij  (ns, ij_count)     background spread (k-1 denominator)
+ *   lat_ij, lon_ij (ij_count)   degrees
+ *   mask_ij  (ij_count)         non-zero = masked, skipped as in :207 */
+int letkf_gpu_set_state(letkf_gpu_t* h, const float* state_ij,
+                        const float* mean_ij, const float* sprd_ij,
+                        const float* lat_ij, const float* lon_ij,
+                        const int32_t* mask_ij);
+
+/* Observations after letkf_obs_read (src/letkf_obs.F90:396-507): the
+ * QC-compacted obs_def fields as separate arrays, obs_hx perturbations
+ * (k, n_obs) and obs_hx_mean (n_obs).  The search structure that replaces
+ * the kd-tree of :491-503 is built on the device by this call. */
+int letkf_gpu_set_obs(letkf_gpu_t* h, int64_t n_obs,
+                      const float* lat, const float* lon, const float* zdim,
+                      const float* time, const float* val, const float* err,
+                      const int32_t* obsid, const int32_t* platid,
+                      const float* hx_pert, const float* hx_mean);
+
+/* State variables (statedef, src/letkf_state.F90:83-98): slab start
+ * (0-based grid_s_idx), levels, ana_inc_max and ana_bounds (2 per var). */
+int letkf_gpu_set_statevars(letkf_gpu_t* h, int32_t nvars,
+                            const int32_t* grid_s_idx, const int32_t* levels,
+                            const float* ana_inc_max, const float* ana_bounds);
+
+/* Localizer (letkf_loc_init + class%init).  Classes other than the ones
+ * above return LETKF_EUNSUPPORTED so the caller keeps its CPU loop. */
+int letkf_gpu_set_localizer(letkf_gpu_t* h, const letkf_loc_desc* desc);
+
+/* Inflation (solver.inflation.{rtps,rtpp,mul}); validated exactly like
+ * letkf_solver_init (src/letkf_solver.F90:114-125). */
+int letkf_gpu_set_solver(letkf_gpu_t* h, float rtps, float rtpp, float mul);
+
+/* Run the local analysis over every gridpoint of the handle.  Inputs stay
+ * unchanged on the device, so run may be repeated. */
+int letkf_gpu_run(letkf_gpu_t* h);
+
+/* Analysis in the reference's single-precision layout:
+ * state_ij (k, ns, ij_count), mean_ij and sprd_ij (ns, ij_count).
+ * Masked points hold the background perturbations, mean and spread. */
+int letkf_gpu_get_state(letkf_gpu_t* h, float* state_ij, float* mean_ij,
+                        float* sprd_ij);
+
+/* The same analysis at the precision it was computed in (FP64 for
+ * LETKF_PREC_FP64; the FP32 results widened otherwise). */
+int letkf_gpu_get_state_f64(letkf_gpu_t* h, double* state_ij,
+                            double* mean_ij, double* sprd_ij);
+
+/* Diagnostics of the last run; maxgroups = letkf_gpu_maxgroups(h). */
+int letkf_gpu_get_diag(letkf_gpu_t* h, const letkf_gpu_diag* diag);
+
+/* localizer_class%maxgroups of the configured localizer. */
+int letkf_gpu_maxgroups(const letkf_gpu_t* h, int32_t* maxgroups);
+
+/* Kept observations of one (gridpoint, group) of the last run: original
+ * obs indices and localization weights, in the library's order.  m_out is
+ * set to the full count even when it exceeds cap. */
+int letkf_gpu_get_selection(letkf_gpu_t* h, int64_t ij, int32_t group,
+                            int32_t* idx_out, float* rloc_out, int64_t cap,
+                            int64_t* m_out);
+
+/* Phase timings and work counts of the last run. */
+int letkf_gpu_get_timing(const letkf_gpu_t* h, letkf_gpu_timing* t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LETKF_GPU_H */
