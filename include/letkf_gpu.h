@@ -127,8 +127,9 @@ typedef struct letkf_gpu_timing {
   double finalize_ms;  /* recenter / clip / inflation / spread      */
   double total_ms;
   int64_t n_problems;  /* (gridpoint, group) solves with m > 0       */
-  int64_t n_pairs;     /* sum of kept observations over all solves   */
+  int64_t n_pairs;     /* kept (gridpoint, group, obs) triples        */
   int64_t n_cand;      /* candidate (gridpoint, obs) pairs tested     */
+  int64_t n_launch;    /* launches per phase (gridpoint chunks)       */
 } letkf_gpu_timing;
 
 typedef struct letkf_gpu_t letkf_gpu_t;

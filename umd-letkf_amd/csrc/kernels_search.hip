@@ -1,0 +1,262 @@
+// kernels_search.hip -- local-observation search and localization.
+//
+// Replaces, per gridpoint, the reference's
+//   maxhz / search radius ........ src/letkf_solver.F90:210-212
+//   kd_search_radius ............. src/letkf_obs.F90:560-573
+//   groups() ..................... letkf_loc_novrt.F90:133-147,
+//                                  letkf_loc_ocean.F90:331-427
+//   localize() + keep r > 1e-4 ... src/letkf_solver.F90:244-268,
+//                                  letkf_loc_novrt.F90:170-183,
+//                                  letkf_loc_ocean.F90:451-506
+//
+// One 64-lane wavefront owns one gridpoint.  The gridpoint's search cap is
+// covered by whole lat/lon cells (a conservative superset computed with
+// margins); lanes stride over the contiguous obs ranges of those cells and
+// apply the exact test (squared chord <= threshold) and the localizer in
+// the deterministic arithmetic of det_math.h.  Kept observations are counted
+// per localization group with ballots.  The same kernel runs twice: the
+// count pass sizes the lists (and writes the diagnostics), the fill pass
+// writes each (gridpoint, group)'s kept (obs index, weight) list in scan
+// order.
+#include <hip/hip_runtime.h>
+
+#include "det_math.h"
+#include "letkf_internal.h"
+
+namespace lk {
+
+constexpr double kRad2Deg = 57.295779513082320876;
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int lane_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// localization weight of one kept-candidate obs for group g (FP32 result)
+__device__ __forceinline__ float loc_weight(const LocDev& L, int g, float hz,
+                                            float d, float Lp, float Ls,
+                                            uint8_t surf, float twp, float tws,
+                                            float zdim, const float* lev_depth) {
+#pragma clang fp contract(off)
+  if (L.cls == LETKF_LOC_OCEAN) {
+    if (surf) {
+      if (g == 1) return 0.0f;
+      float r = dm::gc(d, Ls);
+      if (L.tloc_sat > 0.0f) r = r * tws;
+      return r;
+    }
+    float r = dm::gc(d, Lp);
+    if (L.tloc_prof > 0.0f) r = r * twp;
+    return r;
+  }
+  if (L.cls == LETKF_LOC_VGAUSS) {
+    const double dz = (double)zdim - (double)lev_depth[g];
+    const double sc = (double)L.vt_scale;
+    const double v = dm::exp_neg(-(dz * dz) / (2.0 * sc * sc));
+    return hz * (float)v;
+  }
+  return hz;
+}
+
+__device__ __forceinline__ bool group_has_slabs(const LocDev& L, int g, int mld) {
+  if (L.cls == LETKF_LOC_OCEAN && L.vtloc_surf == 1) return g == 0 || L.max_var_lev > mld;
+  if (L.cls == LETKF_LOC_VGAUSS) return g < L.max_var_lev;
+  return true;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
+                                                VarsDev V, ChunkDev C, DiagDev D) {
+  __shared__ int64_t s_cnt[4][kMaxGroups];
+  __shared__ double s_sum[4][kMaxGroups];
+  __shared__ uint64_t s_hash[4][kMaxGroups];
+  const int wid = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t t = (int64_t)blockIdx.x * 4 + wid;
+  if (t >= C.ngp) return;
+  const int64_t ij = C.i0 + t;
+  const int G = C.G;
+
+  if (lane == 0) {
+    for (int g = 0; g < G; ++g) {
+      s_cnt[wid][g] = FILL ? C.off[t * G + g] : 0;
+      s_sum[wid][g] = 0.0;
+      // fill pass: end of this pair's list (writes never pass it)
+      s_hash[wid][g] = FILL ? (uint64_t)(C.off[t * G + g] + C.cnt[t * G + g]) : 0;
+    }
+  }
+  if (S.mask[ij]) {                                   // src/letkf_solver.F90:207
+    if (!FILL && lane == 0) {
+      for (int g = 0; g < G; ++g) {
+        C.cnt[t * G + g] = 0;
+        C.solve[t * G + g] = 0;
+        D.lg_m[ij * G + g] = 0;
+        D.lg_obsloc[ij * G + g] = 0.0;
+        D.lg_hash[ij * G + g] = 0;
+      }
+      D.obscount[ij] = 0;
+      D.maxhz[ij] = 0.0f;
+      D.mld[ij] = 0;
+    }
+    return;
+  }
+
+  const float lat_ij = S.lat[ij];
+  float Lh = 0.0f, Lp = 0.0f, Ls = 0.0f, maxhz;
+  if (L.cls == LETKF_LOC_OCEAN) {
+    Lp = dm::interp(L.n_prof, L.prof_lat, L.prof_dist, lat_ij);
+    Ls = dm::interp(L.n_sat, L.sat_lat, L.sat_dist, lat_ij);
+    maxhz = Lp > Ls ? Lp : Ls;
+  } else {
+    Lh = dm::interp(L.n_hz, L.hz_lat, L.hz_dist, lat_ij);
+    maxhz = Lh;
+  }
+  float rs;
+  {
+#pragma clang fp contract(off)
+    rs = maxhz * dm::kSTDEV2MAX;                      // src/letkf_solver.F90:211
+  }
+  const double thr2 = dm::thr2(rs);
+
+  int mld = 0;                                        // calc_mldlvl_bkgt
+  if (L.cls == LETKF_LOC_OCEAN && L.vtloc_surf == 1) {
+    const float* mean = S.mean + S.ns * ij;
+    const float t0 = mean[L.bkg_t_var_idx];
+    mld = L.bkg_t_var_len;
+    for (int i = 1; i <= L.bkg_t_var_len - 1; ++i) {
+      const float t1 = mean[L.bkg_t_var_idx + i];
+      if (fabsf(t0 - t1) > L.bkg_t_delta) { mld = i; break; }
+    }
+  }
+
+  const double gx = S.gx[ij], gy = S.gy[ij], gz = S.gz[ij];
+
+  // conservative cell cover of the cap
+  const double theta = (double)rs / dm::kRE * (1.0 + 1e-6) + 1e-9;
+  const double lat0 = (double)lat_ij;
+  double lon0 = (double)S.lon[ij];
+  lon0 = lon0 - 360.0 * floor(lon0 / 360.0);
+  int b0 = 0, b1 = kBands - 1;
+  bool full_lon = true;
+  double dl = 180.0;
+  if (theta < 3.14159) {
+    const double thd = theta * kRad2Deg;
+    b0 = (int)floor(lat0 - thd + 90.0);
+    b1 = (int)floor(lat0 + thd + 90.0);
+    b0 = b0 < 0 ? 0 : (b0 >= kBands ? kBands - 1 : b0);
+    b1 = b1 < 0 ? 0 : (b1 >= kBands ? kBands - 1 : b1);
+    if (lat0 + thd < 90.0 && lat0 - thd > -90.0) {
+      const double sn = sin(theta), cp = cos(lat0 / kRad2Deg);
+      if (sn < cp) {
+        dl = asin(sn / cp) * kRad2Deg * (1.0 + 1e-9) + 1e-6;
+        full_lon = dl >= 180.0;
+      }
+    }
+  }
+  int nr = 1;
+  int rc0[2] = {0, 0}, rc1[2] = {kLonCells - 1, kLonCells - 1};
+  if (!full_lon) {
+    int c0 = (int)floor(lon0 - dl), c1 = (int)floor(lon0 + dl);
+    if (c0 < 0) { nr = 2; rc0[0] = c0 + kLonCells; rc1[0] = kLonCells - 1; rc0[1] = 0; rc1[1] = c1; }
+    else if (c1 >= kLonCells) { nr = 2; rc0[0] = c0; rc1[0] = kLonCells - 1; rc0[1] = 0; rc1[1] = c1 - kLonCells; }
+    else { rc0[0] = c0; rc1[0] = c1; }
+  }
+
+  int64_t nin = 0, ncand = 0;
+  for (int b = b0; b <= b1; ++b) {
+    for (int r = 0; r < nr; ++r) {
+      const int64_t sb = O.cell_start[b * kLonCells + rc0[r]];
+      const int64_t se = O.cell_start[b * kLonCells + rc1[r] + 1];
+      if (!FILL) ncand += se - sb;
+      for (int64_t base = sb; base < se; base += 64) {
+        const int64_t s = base + lane;
+        const bool valid = s < se;
+        double c2 = 9.0;
+        if (valid) c2 = dm::chord2(O.x[s], O.y[s], O.z[s], gx, gy, gz);
+        const bool inside = c2 <= thr2;
+        const uint64_t imask = __ballot(inside);
+        if (imask == 0) continue;
+        nin += __popcll(imask);
+        float d = 0.0f, hz = 0.0f, zd = 0.0f, twp = 1.0f, tws = 1.0f;
+        uint8_t sf = 0;
+        uint64_t mix = 0;
+        if (inside) {
+          d = dm::dist(c2);
+          if (L.cls != LETKF_LOC_OCEAN) hz = dm::gc(d, Lh);
+          if (L.cls == LETKF_LOC_OCEAN) { sf = O.surf[s]; twp = O.tw_prof[s]; tws = O.tw_sat[s]; }
+          if (L.cls == LETKF_LOC_VGAUSS) zd = O.zdim[s];
+          if (!FILL) mix = dm::mix64((uint64_t)O.oidx[s]);
+        }
+        for (int g = 0; g < G; ++g) {
+          float w = 0.0f;
+          if (inside) w = loc_weight(L, g, hz, d, Lp, Ls, sf, twp, tws, zd, V.lev_depth);
+          const bool keep = inside && (w > 1.0e-4f);   // src/letkf_solver.F90:251
+          const uint64_t kmask = __ballot(keep);
+          if (kmask == 0) continue;
+          if (!FILL) {
+            const double sr = wave_sum_f64(keep ? (double)w : 0.0);
+            const uint64_t hs = wave_sum_u64(keep ? mix : 0ull);
+            if (lane == 0) {
+              s_cnt[wid][g] += __popcll(kmask);
+              s_sum[wid][g] += sr;
+              s_hash[wid][g] += hs;
+            }
+          } else {
+            int64_t cur = 0, lim = 0;
+            if (lane == 0) { cur = s_cnt[wid][g]; lim = (int64_t)s_hash[wid][g]; }
+            cur = __shfl(cur, 0, 64);
+            lim = __shfl(lim, 0, 64);
+            if (keep) {
+              const int64_t pos = cur + lane_rank(kmask);
+              if (pos < lim) {
+                C.list_idx[pos] = (int32_t)s;
+                C.list_w[pos] = w;
+              }
+            }
+            if (lane == 0) s_cnt[wid][g] = cur + __popcll(kmask);
+          }
+        }
+      }
+    }
+  }
+
+  if (!FILL && lane == 0) {
+    unsigned long long nsolve = 0;
+    for (int g = 0; g < G; ++g) {
+      const int64_t m = s_cnt[wid][g];
+      nsolve += (m > 0 && group_has_slabs(L, g, mld)) ? 1 : 0;
+      C.cnt[t * G + g] = m;
+      C.solve[t * G + g] = (m > 0 && group_has_slabs(L, g, mld)) ? 1 : 0;
+      D.lg_m[ij * G + g] = (int32_t)m;
+      D.lg_obsloc[ij * G + g] = s_sum[wid][g];
+      D.lg_hash[ij * G + g] = s_hash[wid][g];
+    }
+    D.obscount[ij] = (int32_t)nin;
+    D.maxhz[ij] = maxhz;
+    D.mld[ij] = mld;
+    atomicMax(D.max_count, (int32_t)(nin > 0x7fffffff ? 0x7fffffff : nin));
+    atomicAdd(&D.stats[0], nsolve);
+    atomicAdd(&D.stats[1], (unsigned long long)ncand);
+  }
+}
+
+void launch_search(hipStream_t st, bool fill, const LocDev& L, const ObsDev& O,
+                   const StateDev& S, const VarsDev& V, const ChunkDev& C,
+                   const DiagDev& D) {
+  if (C.ngp <= 0) return;
+  const unsigned nb = (unsigned)((C.ngp + 3) / 4);
+  if (fill) k_search<true><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+  else k_search<false><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+}
+
+}  // namespace lk

@@ -1,0 +1,740 @@
+// letkf_gpu.hip -- C ABI (include/letkf_gpu.h) and host orchestration.
+//
+// letkf_gpu_run() replaces letkf_solver_run (src/letkf_solver.F90:150-389).
+// Gridpoints are processed in chunks so that the per-(gridpoint, group)
+// work arrays stay bounded; each chunk runs
+//   search count -> offsets scan -> search fill -> products -> eigensolve
+//   and application -> post-processing
+// on the handle's stream.  Inputs stay untouched on the device, so a run can
+// be repeated (benchmarks, cycling).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "det_math.h"
+#include "letkf_internal.h"
+
+using namespace lk;
+
+namespace lk {
+size_t eigen_lds_bytes(int k);
+}
+
+struct letkf_gpu_t {
+  letkf_gpu_dims dims{};
+  int64_t k = 0, ns = 0, nij = 0;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string err;
+  bool have_state = false, have_obs = false, have_vars = false, have_loc = false;
+  bool have_solver = false, have_run = false;
+
+  // background
+  float *d_state = nullptr, *d_mean = nullptr, *d_sprd = nullptr;
+  float *d_lat = nullptr, *d_lon = nullptr;
+  int32_t* d_mask = nullptr;
+  double *d_gx = nullptr, *d_gy = nullptr, *d_gz = nullptr;
+
+  // observations (raw copies, then sorted)
+  int64_t nobs = 0;
+  double *d_ox = nullptr, *d_oy = nullptr, *d_oz = nullptr;
+  float *d_ozdim = nullptr, *d_oerr = nullptr, *d_odep = nullptr, *d_otime = nullptr;
+  int32_t *d_oidx = nullptr, *d_oobsid = nullptr, *d_oplatid = nullptr;
+  float* d_ohx = nullptr;
+  int32_t* d_cell_start = nullptr;
+  uint8_t* d_osurf = nullptr;
+  float *d_otwp = nullptr, *d_otws = nullptr;
+
+  // variables / localizer / solver
+  int nvars = 0;
+  std::vector<int32_t> var_s, var_lev;
+  int32_t *d_slab_var = nullptr, *d_slab_lev = nullptr;
+  float *d_inc_max = nullptr, *d_bounds = nullptr;
+  LocDev loc{};
+  std::vector<int32_t> sat_obs, sat_plats;
+  float* d_lev_depth = nullptr;
+  float rtps = 0.0f, rtpp = 0.0f, mul = 1.0f;
+  int G = 1;
+
+  // outputs
+  double *d_ana = nullptr, *d_amean = nullptr, *d_asprd = nullptr;
+  float* d_maxhz = nullptr;
+  int32_t *d_obscount = nullptr, *d_lgm = nullptr, *d_mld = nullptr, *d_maxcount = nullptr;
+  unsigned long long* d_stats = nullptr;
+  double* d_lgloc = nullptr;
+  uint64_t* d_lghash = nullptr;
+
+  // chunk work
+  int64_t chunk_gp = 0;
+  int64_t alloc_G = 0;
+  int64_t* d_cnt = nullptr;
+  int64_t* d_off = nullptr;
+  uint8_t* d_solve = nullptr;
+  int32_t* d_list_idx = nullptr;
+  float* d_list_w = nullptr;
+  int64_t list_cap = 0;
+  double *d_B = nullptr, *d_b = nullptr;
+  void* d_tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int64_t* h_tot = nullptr;    // pinned
+
+  letkf_gpu_timing timing{};
+};
+
+static std::string g_create_err;
+
+static int fail(letkf_gpu_t* h, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf; else g_create_err = buf;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                        \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail((h), e_ == hipErrorOutOfMemory ? LETKF_ENOMEM : LETKF_EHIP,  \
+                  "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,    \
+                  __LINE__);                                                   \
+  } while (0)
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t n) {
+  if (*p) { hipFree(*p); *p = nullptr; }
+  if (n == 0) n = 1;
+  return hipMalloc((void**)p, n * sizeof(T));
+}
+template <typename T>
+static void dfree(T*& p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+extern "C" {
+
+const char* letkf_gpu_version(void) { return "letkf_gpu 0.1 (gfx950)"; }
+
+const char* letkf_gpu_last_error(const letkf_gpu_t* h) {
+  return h ? h->err.c_str() : g_create_err.c_str();
+}
+
+int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
+  if (!d || !out) return fail(nullptr, LETKF_EINVAL, "null argument");
+  *out = nullptr;
+  if (d->ens_size < 2 || d->ens_size > kMaxK - 1)
+    return fail(nullptr, LETKF_EUNSUPPORTED, "ens_size %lld outside [2, %d]",
+                (long long)d->ens_size, kMaxK - 1);
+  if (d->grid_ns < 1 || d->ij_count < 0)
+    return fail(nullptr, LETKF_EINVAL, "invalid grid_ns / ij_count");
+  if (d->precision < LETKF_PREC_FP64 || d->precision > LETKF_PREC_BF16)
+    return fail(nullptr, LETKF_EINVAL, "invalid precision %d", d->precision);
+  if (d->path != LETKF_PATH_PLAIN && d->path != LETKF_PATH_BATCHED)
+    return fail(nullptr, LETKF_EINVAL, "invalid path %d", d->path);
+  if (d->path == LETKF_PATH_PLAIN && d->precision != LETKF_PREC_FP64)
+    return fail(nullptr, LETKF_EUNSUPPORTED, "the plain path computes in FP64 only");
+  if (d->path == LETKF_PATH_BATCHED)
+    return fail(nullptr, LETKF_EUNSUPPORTED, "batched path not available in this build");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, LETKF_EHIP, "no HIP device");
+  letkf_gpu_t* h = new letkf_gpu_t();
+  h->dims = *d;
+  if (h->dims.max_obs <= 0) h->dims.max_obs = 100000;   // src/letkf_solver.F90:165
+  h->k = d->ens_size;
+  h->ns = d->grid_ns;
+  h->nij = d->ij_count;
+  hipGetDevice(&h->device);
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(nullptr, LETKF_EHIP, "hipStreamCreate failed");
+  }
+  if (hipHostMalloc((void**)&h->h_tot, 4 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
+    hipStreamDestroy(h->stream);
+    delete h;
+    return fail(nullptr, LETKF_ENOMEM, "pinned allocation failed");
+  }
+  *out = h;
+  return LETKF_OK;
+}
+
+int letkf_gpu_destroy(letkf_gpu_t* h) {
+  if (!h) return LETKF_OK;
+  hipStreamSynchronize(h->stream);
+  dfree(h->d_state); dfree(h->d_mean); dfree(h->d_sprd); dfree(h->d_lat); dfree(h->d_lon);
+  dfree(h->d_mask); dfree(h->d_gx); dfree(h->d_gy); dfree(h->d_gz);
+  dfree(h->d_ox); dfree(h->d_oy); dfree(h->d_oz); dfree(h->d_ozdim); dfree(h->d_oerr);
+  dfree(h->d_odep); dfree(h->d_otime); dfree(h->d_oidx); dfree(h->d_oobsid);
+  dfree(h->d_oplatid); dfree(h->d_ohx); dfree(h->d_cell_start); dfree(h->d_osurf);
+  dfree(h->d_otwp); dfree(h->d_otws);
+  dfree(h->d_slab_var); dfree(h->d_slab_lev); dfree(h->d_inc_max); dfree(h->d_bounds);
+  dfree(h->d_lev_depth);
+  dfree(h->d_ana); dfree(h->d_amean); dfree(h->d_asprd); dfree(h->d_maxhz);
+  dfree(h->d_obscount); dfree(h->d_lgm); dfree(h->d_mld); dfree(h->d_maxcount); dfree(h->d_stats);
+  dfree(h->d_lgloc); dfree(h->d_lghash);
+  dfree(h->d_cnt); dfree(h->d_off); dfree(h->d_solve); dfree(h->d_list_idx);
+  dfree(h->d_list_w); dfree(h->d_B); dfree(h->d_b); dfree(h->d_tmp);
+  if (h->h_tot) hipHostFree(h->h_tot);
+  hipStreamDestroy(h->stream);
+  delete h;
+  return LETKF_OK;
+}
+
+int letkf_gpu_set_state(letkf_gpu_t* h, const float* state_ij, const float* mean_ij,
+                        const float* sprd_ij, const float* lat_ij, const float* lon_ij,
+                        const int32_t* mask_ij) {
+  if (!h) return LETKF_EINVAL;
+  const int64_t nij = h->nij, k = h->k, ns = h->ns;
+  if (nij > 0 && (!state_ij || !mean_ij || !sprd_ij || !lat_ij || !lon_ij || !mask_ij))
+    return fail(h, LETKF_EINVAL, "null state array");
+  HIPCHK(h, dalloc(&h->d_state, (size_t)(k * ns * nij)));
+  HIPCHK(h, dalloc(&h->d_mean, (size_t)(ns * nij)));
+  HIPCHK(h, dalloc(&h->d_sprd, (size_t)(ns * nij)));
+  HIPCHK(h, dalloc(&h->d_lat, (size_t)nij));
+  HIPCHK(h, dalloc(&h->d_lon, (size_t)nij));
+  HIPCHK(h, dalloc(&h->d_mask, (size_t)nij));
+  HIPCHK(h, dalloc(&h->d_gx, (size_t)nij));
+  HIPCHK(h, dalloc(&h->d_gy, (size_t)nij));
+  HIPCHK(h, dalloc(&h->d_gz, (size_t)nij));
+  if (nij > 0) {
+    HIPCHK(h, hipMemcpyAsync(h->d_state, state_ij, sizeof(float) * k * ns * nij, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_mean, mean_ij, sizeof(float) * ns * nij, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_sprd, sprd_ij, sizeof(float) * ns * nij, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_lat, lat_ij, sizeof(float) * nij, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_lon, lon_ij, sizeof(float) * nij, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_mask, mask_ij, sizeof(int32_t) * nij, hipMemcpyHostToDevice, h->stream));
+    launch_grid_xyz(h->stream, nij, h->d_lat, h->d_lon, h->d_gx, h->d_gy, h->d_gz);
+    HIPCHK(h, hipGetLastError());
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->have_state = true;
+  return LETKF_OK;
+}
+
+static int upload_flags(letkf_gpu_t* h);
+
+int letkf_gpu_set_obs(letkf_gpu_t* h, int64_t n, const float* lat, const float* lon,
+                      const float* zdim, const float* time, const float* val,
+                      const float* err, const int32_t* obsid, const int32_t* platid,
+                      const float* hx_pert, const float* hx_mean) {
+  if (!h) return LETKF_EINVAL;
+  if (n < 0) return fail(h, LETKF_EINVAL, "negative n_obs");
+  if (n > 0x7ffffff0) return fail(h, LETKF_EUNSUPPORTED, "n_obs above 2^31");
+  if (n > 0 && (!lat || !lon || !zdim || !time || !val || !err || !obsid || !platid ||
+                !hx_pert || !hx_mean))
+    return fail(h, LETKF_EINVAL, "null observation array");
+  const int64_t k = h->k;
+  h->nobs = n;
+  // raw uploads (freed at the end of the call)
+  float *r_lat = nullptr, *r_lon = nullptr, *r_zdim = nullptr, *r_time = nullptr;
+  float *r_val = nullptr, *r_err = nullptr, *r_hx = nullptr, *r_hxm = nullptr;
+  int32_t *r_obsid = nullptr, *r_platid = nullptr, *cell = nullptr, *scell = nullptr;
+  int32_t *iota = nullptr, *perm = nullptr;
+  void* tmp = nullptr;
+  int rc = LETKF_OK;
+  auto cleanup = [&]() {
+    dfree(r_lat); dfree(r_lon); dfree(r_zdim); dfree(r_time); dfree(r_val); dfree(r_err);
+    dfree(r_hx); dfree(r_hxm); dfree(r_obsid); dfree(r_platid); dfree(cell); dfree(scell);
+    dfree(iota); dfree(perm);
+    if (tmp) hipFree(tmp);
+  };
+#define OBSCHK(call)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      rc = fail(h, e_ == hipErrorOutOfMemory ? LETKF_ENOMEM : LETKF_EHIP,      \
+                "%s: %s", #call, hipGetErrorString(e_));                       \
+      cleanup();                                                               \
+      return rc;                                                               \
+    }                                                                          \
+  } while (0)
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  OBSCHK(dalloc(&h->d_ox, nn)); OBSCHK(dalloc(&h->d_oy, nn)); OBSCHK(dalloc(&h->d_oz, nn));
+  OBSCHK(dalloc(&h->d_ozdim, nn)); OBSCHK(dalloc(&h->d_oerr, nn)); OBSCHK(dalloc(&h->d_odep, nn));
+  OBSCHK(dalloc(&h->d_otime, nn)); OBSCHK(dalloc(&h->d_oidx, nn)); OBSCHK(dalloc(&h->d_oobsid, nn));
+  OBSCHK(dalloc(&h->d_oplatid, nn)); OBSCHK(dalloc(&h->d_ohx, nn * k));
+  OBSCHK(dalloc(&h->d_cell_start, (size_t)kCells + 1));
+  OBSCHK(dalloc(&h->d_osurf, nn)); OBSCHK(dalloc(&h->d_otwp, nn)); OBSCHK(dalloc(&h->d_otws, nn));
+  OBSCHK(dalloc(&r_lat, nn)); OBSCHK(dalloc(&r_lon, nn)); OBSCHK(dalloc(&r_zdim, nn));
+  OBSCHK(dalloc(&r_time, nn)); OBSCHK(dalloc(&r_val, nn)); OBSCHK(dalloc(&r_err, nn));
+  OBSCHK(dalloc(&r_hx, nn * k)); OBSCHK(dalloc(&r_hxm, nn)); OBSCHK(dalloc(&r_obsid, nn));
+  OBSCHK(dalloc(&r_platid, nn)); OBSCHK(dalloc(&cell, nn)); OBSCHK(dalloc(&scell, nn));
+  OBSCHK(dalloc(&iota, nn)); OBSCHK(dalloc(&perm, nn));
+  hipStream_t st = h->stream;
+  if (n > 0) {
+    OBSCHK(hipMemcpyAsync(r_lat, lat, 4 * n, hipMemcpyHostToDevice, st));
+    OBSCHK(hipMemcpyAsync(r_lon, lon, 4 * n, hipMemcpyHostToDevice, st));
+    OBSCHK(hipMemcpyAsync(r_zdim, zdim, 4 * n, hipMemcpyHostToDevice, st));
+    OBSCHK(hipMemcpyAsync(r_time, time, 4 * n, hipMemcpyHostToDevice, st));
+    OBSCHK(hipMemcpyAsync(r_val, val, 4 * n, hipMemcpyHostToDevice, st));
+    OBSCHK(hipMemcpyAsync(r_err, err, 4 * n, hipMemcpyHostToDevice, st));
+    OBSCHK(hipMemcpyAsync(r_hx, hx_pert, 4 * n * k, hipMemcpyHostToDevice, st));
+    OBSCHK(hipMemcpyAsync(r_hxm, hx_mean, 4 * n, hipMemcpyHostToDevice, st));
+    OBSCHK(hipMemcpyAsync(r_obsid, obsid, 4 * n, hipMemcpyHostToDevice, st));
+    OBSCHK(hipMemcpyAsync(r_platid, platid, 4 * n, hipMemcpyHostToDevice, st));
+    launch_obs_prep(st, n, (int)k, r_lat, r_lon, cell, iota);
+    OBSCHK(hipGetLastError());
+    size_t tb = 0;
+    OBSCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, cell, scell, iota, perm, (int)n, 0, 17, st));
+    OBSCHK(hipMalloc(&tmp, tb));
+    OBSCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, cell, scell, iota, perm, (int)n, 0, 17, st));
+  }
+  launch_obs_gather(st, n, (int)k, perm, r_lat, r_lon, r_zdim, r_time, r_val, r_err,
+                    r_obsid, r_platid, r_hx, r_hxm, scell, h->d_ox, h->d_oy, h->d_oz,
+                    h->d_ozdim, h->d_oerr, h->d_odep, h->d_oidx, h->d_oobsid,
+                    h->d_oplatid, h->d_otime, h->d_ohx, h->d_cell_start);
+  OBSCHK(hipGetLastError());
+  OBSCHK(hipStreamSynchronize(st));
+  cleanup();
+#undef OBSCHK
+  h->have_obs = true;
+  if (h->have_loc) return upload_flags(h);
+  return LETKF_OK;
+}
+
+static void build_interp(const letkf_linearinterp_lat& c, int* n, float* lat, float* dist) {
+  // linearinterp_lat_init, src/letkf_loc.F90:262-293
+  int m = 0;
+  float l = 0.0f, r = 0.0f;
+  for (int i = 0; i < c.npoints; ++i) {
+    l = c.lat[i];
+    r = c.radius[i];
+    if (l > 0.0f && m == 0) { lat[m] = 0.0f; dist[m] = r; ++m; }
+    lat[m] = l; dist[m] = r; ++m;
+  }
+  if (l < 90.0f) { lat[m] = 90.0f; dist[m] = r; ++m; }
+  *n = c.npoints > 0 ? m : 0;
+}
+
+static int rebuild_slabs(letkf_gpu_t* h) {
+  std::vector<int32_t> sv(h->ns, -1), sl(h->ns, 0);
+  int maxlev = 1;
+  for (int v = 0; v < h->nvars; ++v) {
+    maxlev = std::max(maxlev, (int)h->var_lev[v]);
+    for (int j = 0; j < h->var_lev[v]; ++j) {
+      sv[h->var_s[v] + j] = v;
+      sl[h->var_s[v] + j] = j;
+    }
+  }
+  h->loc.max_var_lev = maxlev;
+  HIPCHK(h, dalloc(&h->d_slab_var, (size_t)h->ns));
+  HIPCHK(h, dalloc(&h->d_slab_lev, (size_t)h->ns));
+  HIPCHK(h, hipMemcpy(h->d_slab_var, sv.data(), 4 * h->ns, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->d_slab_lev, sl.data(), 4 * h->ns, hipMemcpyHostToDevice));
+  return LETKF_OK;
+}
+
+int letkf_gpu_set_statevars(letkf_gpu_t* h, int32_t nvars, const int32_t* s_idx,
+                            const int32_t* levels, const float* inc_max,
+                            const float* bounds) {
+  if (!h) return LETKF_EINVAL;
+  if (nvars < 0 || (nvars > 0 && (!s_idx || !levels || !inc_max || !bounds)))
+    return fail(h, LETKF_EINVAL, "invalid statevars");
+  for (int v = 0; v < nvars; ++v) {
+    if (levels[v] < 1 || s_idx[v] < 0 || s_idx[v] + levels[v] > h->ns)
+      return fail(h, LETKF_EINVAL, "variable %d: slabs [%d, %d) outside grid_ns %lld", v,
+                  s_idx[v], s_idx[v] + levels[v], (long long)h->ns);
+  }
+  h->nvars = nvars;
+  h->var_s.assign(s_idx, s_idx + nvars);
+  h->var_lev.assign(levels, levels + nvars);
+  HIPCHK(h, dalloc(&h->d_inc_max, (size_t)std::max(nvars, 1)));
+  HIPCHK(h, dalloc(&h->d_bounds, (size_t)std::max(2 * nvars, 1)));
+  if (nvars > 0) {
+    HIPCHK(h, hipMemcpy(h->d_inc_max, inc_max, 4 * nvars, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_bounds, bounds, 8 * nvars, hipMemcpyHostToDevice));
+  }
+  int rc = rebuild_slabs(h);
+  if (rc) return rc;
+  h->have_vars = true;
+  return LETKF_OK;
+}
+
+static int upload_flags(letkf_gpu_t* h) {
+  int32_t *so = nullptr, *sp = nullptr;
+  HIPCHK(h, dalloc(&so, std::max<size_t>(h->sat_obs.size(), 1)));
+  HIPCHK(h, dalloc(&sp, std::max<size_t>(h->sat_plats.size(), 1)));
+  if (!h->sat_obs.empty())
+    HIPCHK(h, hipMemcpy(so, h->sat_obs.data(), 4 * h->sat_obs.size(), hipMemcpyHostToDevice));
+  if (!h->sat_plats.empty())
+    HIPCHK(h, hipMemcpy(sp, h->sat_plats.data(), 4 * h->sat_plats.size(), hipMemcpyHostToDevice));
+  launch_obs_flags(h->stream, h->loc, h->nobs, h->d_oobsid, h->d_oplatid, h->d_otime, so,
+                   (int)h->sat_obs.size(), sp, (int)h->sat_plats.size(), h->d_osurf,
+                   h->d_otwp, h->d_otws);
+  hipError_t e = hipGetLastError();
+  hipStreamSynchronize(h->stream);
+  hipFree(so);
+  hipFree(sp);
+  if (e != hipSuccess) return fail(h, LETKF_EHIP, "obs flags: %s", hipGetErrorString(e));
+  return LETKF_OK;
+}
+
+int letkf_gpu_set_localizer(letkf_gpu_t* h, const letkf_loc_desc* d) {
+  if (!h || !d) return LETKF_EINVAL;
+  LocDev L{};
+  L.cls = d->cls;
+  switch (d->cls) {
+    case LETKF_LOC_NOVRT:
+      L.maxgroups = 1;
+      break;
+    case LETKF_LOC_OCEAN:
+      if (d->vtloc_surf != 0 && d->vtloc_surf != 1)
+        return fail(h, LETKF_EINVAL, "illegal vtloc_surf type %d", d->vtloc_surf);
+      L.maxgroups = d->vtloc_surf == 1 ? 2 : 1;
+      break;
+    case LETKF_LOC_VGAUSS:
+      if (d->nlev < 1 || !d->lev_depth || !(d->vt_scale > 0.0f))
+        return fail(h, LETKF_EINVAL, "LOC_VGAUSS needs nlev >= 1, lev_depth and vt_scale > 0");
+      L.maxgroups = d->nlev;
+      break;
+    default:
+      return fail(h, LETKF_EUNSUPPORTED, "localizer class %d has no GPU implementation", d->cls);
+  }
+  if (L.maxgroups > kMaxGroups)
+    return fail(h, LETKF_EUNSUPPORTED, "maxgroups %d above %d", L.maxgroups, kMaxGroups);
+  auto chk_tab = [&](const letkf_linearinterp_lat& t, const char* name) -> int {
+    if (t.npoints < 1 || t.npoints > LETKF_MAX_INTERP)
+      return fail(h, LETKF_EINVAL, "%s: linearinterp_lat needs 1..%d points", name, LETKF_MAX_INTERP);
+    return LETKF_OK;
+  };
+  int rc;
+  if (d->cls == LETKF_LOC_OCEAN) {
+    if ((rc = chk_tab(d->hzloc_prof, "hzloc_prof"))) return rc;
+    if ((rc = chk_tab(d->hzloc_sat, "hzloc_sat"))) return rc;
+    if (d->vtloc_surf == 1 && (d->bkg_t_var_idx < 0 || d->bkg_t_var_len < 1 ||
+                               d->bkg_t_var_idx + d->bkg_t_var_len > h->ns))
+      return fail(h, LETKF_EINVAL, "bkg_t_var slabs outside the state");
+    if (d->n_sat_obs < 0 || d->n_sat_obs > LETKF_MAX_IDLIST || d->n_sat_plats < 0 ||
+        d->n_sat_plats > LETKF_MAX_IDLIST)
+      return fail(h, LETKF_EINVAL, "sat_obs / sat_plats list too long");
+  } else {
+    if ((rc = chk_tab(d->hzloc, "hzloc"))) return rc;
+  }
+  build_interp(d->hzloc, &L.n_hz, L.hz_lat, L.hz_dist);
+  build_interp(d->hzloc_prof, &L.n_prof, L.prof_lat, L.prof_dist);
+  build_interp(d->hzloc_sat, &L.n_sat, L.sat_lat, L.sat_dist);
+  L.tloc_prof = d->tloc_prof;
+  L.tloc_sat = d->tloc_sat;
+  L.vtloc_surf = d->vtloc_surf;
+  L.bkg_t_delta = d->bkg_t_delta;
+  L.bkg_t_var_idx = d->bkg_t_var_idx;
+  L.bkg_t_var_len = d->bkg_t_var_len;
+  L.vt_scale = d->vt_scale;
+  L.nlev = d->cls == LETKF_LOC_VGAUSS ? d->nlev : 0;
+  L.max_var_lev = h->loc.max_var_lev > 0 ? h->loc.max_var_lev : 1;
+  h->sat_obs.assign(d->sat_obs, d->sat_obs + (d->cls == LETKF_LOC_OCEAN ? d->n_sat_obs : 0));
+  h->sat_plats.assign(d->sat_plats, d->sat_plats + (d->cls == LETKF_LOC_OCEAN ? d->n_sat_plats : 0));
+  if (d->cls == LETKF_LOC_VGAUSS) {
+    HIPCHK(h, dalloc(&h->d_lev_depth, (size_t)d->nlev));
+    HIPCHK(h, hipMemcpy(h->d_lev_depth, d->lev_depth, 4 * d->nlev, hipMemcpyHostToDevice));
+  }
+  h->loc = L;
+  h->G = L.maxgroups;
+  h->have_loc = true;
+  if (h->have_vars) { rc = rebuild_slabs(h); if (rc) return rc; }
+  if (h->have_obs) return upload_flags(h);
+  return LETKF_OK;
+}
+
+int letkf_gpu_set_solver(letkf_gpu_t* h, float rtps, float rtpp, float mul) {
+  if (!h) return LETKF_EINVAL;
+  // src/letkf_solver.F90:114-125
+  if (rtps > 1.0f || rtps < 0.0f)
+    return fail(h, LETKF_EINVAL, "illegal value for infl_rtps (should be <1.0 and >0.0).");
+  if (rtpp > 1.0f || rtpp < 0.0f)
+    return fail(h, LETKF_EINVAL, "illegal value for infl_rtpp (should be <1.0 and >0.0).");
+  if (rtpp != 0.0f && rtps != 0.0f)
+    return fail(h, LETKF_EINVAL, "cannot have both RTPS and RTPP enabled at the same time. ");
+  if (mul < 1.0f) return fail(h, LETKF_EINVAL, "infl_mul must be >=1.0.");
+  h->rtps = rtps;
+  h->rtpp = rtpp;
+  h->mul = mul;
+  h->have_solver = true;
+  return LETKF_OK;
+}
+
+int letkf_gpu_maxgroups(const letkf_gpu_t* h, int32_t* g) {
+  if (!h || !g) return LETKF_EINVAL;
+  if (!h->have_loc) return LETKF_ESTATE;
+  *g = h->G;
+  return LETKF_OK;
+}
+
+static int ensure_outputs(letkf_gpu_t* h) {
+  const int64_t k = h->k, ns = h->ns, nij = h->nij, G = h->G;
+  if (!h->d_ana) {
+    HIPCHK(h, dalloc(&h->d_ana, (size_t)(k * ns * nij)));
+    HIPCHK(h, dalloc(&h->d_amean, (size_t)(ns * nij)));
+    HIPCHK(h, dalloc(&h->d_asprd, (size_t)(ns * nij)));
+    HIPCHK(h, dalloc(&h->d_maxhz, (size_t)nij));
+    HIPCHK(h, dalloc(&h->d_obscount, (size_t)nij));
+    HIPCHK(h, dalloc(&h->d_mld, (size_t)nij));
+    HIPCHK(h, dalloc(&h->d_maxcount, 1));
+    HIPCHK(h, dalloc(&h->d_stats, 2));
+  }
+  if (h->alloc_G == G) return LETKF_OK;
+  HIPCHK(h, dalloc(&h->d_lgm, (size_t)(G * nij)));
+  HIPCHK(h, dalloc(&h->d_lgloc, (size_t)(G * nij)));
+  HIPCHK(h, dalloc(&h->d_lghash, (size_t)(G * nij)));
+  // chunk sizing: bound the per-pair FP64 matrices to about 4 GB
+  int64_t pairs = std::max<int64_t>(1, std::min<int64_t>(131072, (4ll << 30) / (8 * k * k)));
+  int64_t cg = std::max<int64_t>(1, pairs / G);
+  cg = std::min<int64_t>(cg, std::max<int64_t>(nij, 1));
+  h->chunk_gp = cg;
+  const int64_t np = cg * G;
+  HIPCHK(h, dalloc(&h->d_cnt, (size_t)np));
+  HIPCHK(h, dalloc(&h->d_off, (size_t)np));
+  HIPCHK(h, dalloc(&h->d_solve, (size_t)np));
+  HIPCHK(h, dalloc(&h->d_B, (size_t)(np * k * k)));
+  HIPCHK(h, dalloc(&h->d_b, (size_t)(np * k)));
+  size_t tb = 0;
+  HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, h->d_cnt, h->d_off, (int)np, h->stream));
+  HIPCHK(h, dalloc((char**)&h->d_tmp, tb));
+  h->tmp_bytes = tb;
+  h->alloc_G = G;
+  return LETKF_OK;
+}
+
+int letkf_gpu_run(letkf_gpu_t* h) {
+  if (!h) return LETKF_EINVAL;
+  if (!h->have_state || !h->have_obs || !h->have_loc || !h->have_solver || !h->have_vars)
+    return fail(h, LETKF_ESTATE, "set_state, set_obs, set_statevars, set_localizer and "
+                                 "set_solver must precede run");
+  if (h->loc.cls == LETKF_LOC_OCEAN && h->loc.vtloc_surf == 1 && h->nvars == 0)
+    return fail(h, LETKF_EINVAL, "LOC_OCEAN bkg_t needs state variables");
+  if (h->loc.cls == LETKF_LOC_VGAUSS && h->loc.max_var_lev > h->loc.nlev)
+    return fail(h, LETKF_EINVAL, "LOC_VGAUSS: a variable has %d levels but the localizer "
+                "has %d", h->loc.max_var_lev, h->loc.nlev);
+  if (h->ns > 2048)
+    return fail(h, LETKF_EUNSUPPORTED, "grid_ns %lld above 2048", (long long)h->ns);
+  if (eigen_lds_bytes((int)h->k) + 8192 + 64 > 150 * 1024)
+    return fail(h, LETKF_EUNSUPPORTED, "ens_size %lld too large for the FP64 eigensolver",
+                (long long)h->k);
+  int rc = ensure_outputs(h);
+  if (rc) return rc;
+  hipStream_t st = h->stream;
+  const int64_t k = h->k, G = h->G;
+
+  StateDev S{h->k, h->ns, h->nij, h->d_state, h->d_mean, h->d_sprd, h->d_lat, h->d_lon,
+             h->d_mask, h->d_gx, h->d_gy, h->d_gz};
+  ObsDev O{h->nobs, h->d_ox, h->d_oy, h->d_oz, h->d_ozdim, h->d_oerr, h->d_odep,
+           h->d_oidx, h->d_osurf, h->d_otwp, h->d_otws, h->d_ohx, h->d_cell_start};
+  VarsDev V{h->nvars, h->d_slab_var, h->d_slab_lev, h->d_inc_max, h->d_bounds, h->d_lev_depth};
+  DiagDev D{h->d_maxhz, h->d_obscount, h->d_lgm, h->d_lgloc, h->d_lghash, h->d_mld,
+            h->d_maxcount, h->d_stats};
+  HIPCHK(h, hipMemsetAsync(h->d_maxcount, 0, sizeof(int32_t), st));
+  HIPCHK(h, hipMemsetAsync(h->d_stats, 0, 2 * sizeof(unsigned long long), st));
+
+  std::vector<hipEvent_t> ev;
+  auto rec = [&]() -> int {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    hipEventRecord(e, st);
+    ev.push_back(e);
+    return 0;
+  };
+  letkf_gpu_timing T{};
+  rec();
+  for (int64_t i0 = 0; i0 < h->nij; i0 += h->chunk_gp) {
+    const int64_t ngp = std::min(h->chunk_gp, h->nij - i0);
+    const int64_t np = ngp * G;
+    ChunkDev C{i0, ngp, (int)G, h->d_cnt, h->d_off, h->d_solve, h->d_list_idx, h->d_list_w};
+    launch_search(st, false, h->loc, O, S, V, C, D);
+    HIPCHK(h, hipGetLastError());
+    size_t tb = h->tmp_bytes;
+    HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_tmp, tb, h->d_cnt, h->d_off, (int)np, st));
+    HIPCHK(h, hipMemcpyAsync(h->h_tot, h->d_off + (np - 1), 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(h->h_tot + 1, h->d_cnt + (np - 1), 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(h->h_tot + 2, h->d_maxcount, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    const int64_t total = h->h_tot[0] + h->h_tot[1];
+    const int32_t maxc = (int32_t)(h->h_tot[2] & 0xffffffff);
+    if (maxc > h->dims.max_obs)
+      return fail(h, LETKF_ECAPACITY, "a gridpoint has %d observations in its search radius, "
+                  "above max_obs = %lld", maxc, (long long)h->dims.max_obs);
+    if (total > h->list_cap) {
+      const int64_t cap = total + total / 4 + 1024;
+      HIPCHK(h, dalloc(&h->d_list_idx, (size_t)cap));
+      HIPCHK(h, dalloc(&h->d_list_w, (size_t)cap));
+      h->list_cap = cap;
+      C.list_idx = h->d_list_idx;
+      C.list_w = h->d_list_w;
+    }
+    T.n_pairs += total;
+    rec();
+    launch_search(st, true, h->loc, O, S, V, C, D);
+    HIPCHK(h, hipGetLastError());
+    rec();
+    launch_product_f64(st, O, S, C, h->d_B, h->d_b);
+    HIPCHK(h, hipGetLastError());
+    rec();
+    launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_ana);
+    HIPCHK(h, hipGetLastError());
+    rec();
+    launch_finalize(st, h->loc, S, V, C, D, h->rtps, h->rtpp, h->mul, h->d_ana, h->d_ana,
+                    h->d_amean, h->d_asprd);
+    HIPCHK(h, hipGetLastError());
+    rec();
+    (void)k;
+  }
+  {
+    unsigned long long stt[2] = {0, 0};
+    HIPCHK(h, hipMemcpyAsync(stt, h->d_stats, sizeof stt, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    T.n_problems = (int64_t)stt[0];
+    T.n_cand = (int64_t)stt[1];
+  }
+  // events: start, then per chunk [count+scan, fill, product, eigen, finalize]
+  for (size_t c = 0; c + 5 < ev.size(); c += 5) {
+    float a, b, p, e, f;
+    hipEventElapsedTime(&a, ev[c], ev[c + 1]);
+    hipEventElapsedTime(&b, ev[c + 1], ev[c + 2]);
+    hipEventElapsedTime(&p, ev[c + 2], ev[c + 3]);
+    hipEventElapsedTime(&e, ev[c + 3], ev[c + 4]);
+    hipEventElapsedTime(&f, ev[c + 4], ev[c + 5]);
+    T.search_ms += a + b;
+    T.product_ms += p;
+    T.eigen_ms += e;
+    T.finalize_ms += f;
+    T.n_launch += 1;
+  }
+  if (ev.size() >= 2) {
+    float tt;
+    hipEventElapsedTime(&tt, ev.front(), ev.back());
+    T.total_ms = tt;
+  }
+  for (auto e : ev) hipEventDestroy(e);
+  h->timing = T;
+  h->have_run = true;
+  return LETKF_OK;
+}
+
+int letkf_gpu_get_state_f64(letkf_gpu_t* h, double* state, double* mean, double* sprd) {
+  if (!h) return LETKF_EINVAL;
+  if (!h->have_run) return fail(h, LETKF_ESTATE, "no analysis: call letkf_gpu_run first");
+  const int64_t k = h->k, ns = h->ns, nij = h->nij;
+  if (state) HIPCHK(h, hipMemcpy(state, h->d_ana, 8 * k * ns * nij, hipMemcpyDeviceToHost));
+  if (mean) HIPCHK(h, hipMemcpy(mean, h->d_amean, 8 * ns * nij, hipMemcpyDeviceToHost));
+  if (sprd) HIPCHK(h, hipMemcpy(sprd, h->d_asprd, 8 * ns * nij, hipMemcpyDeviceToHost));
+  return LETKF_OK;
+}
+
+int letkf_gpu_get_state(letkf_gpu_t* h, float* state, float* mean, float* sprd) {
+  if (!h) return LETKF_EINVAL;
+  if (!h->have_run) return fail(h, LETKF_ESTATE, "no analysis: call letkf_gpu_run first");
+  const int64_t k = h->k, ns = h->ns, nij = h->nij;
+  std::vector<double> buf;
+  auto conv = [&](double* src, float* dst, int64_t n) -> int {
+    buf.resize((size_t)n);
+    HIPCHK(h, hipMemcpy(buf.data(), src, 8 * n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) dst[i] = (float)buf[i];
+    return LETKF_OK;
+  };
+  int rc;
+  if (state && (rc = conv(h->d_ana, state, k * ns * nij))) return rc;
+  if (mean && (rc = conv(h->d_amean, mean, ns * nij))) return rc;
+  if (sprd && (rc = conv(h->d_asprd, sprd, ns * nij))) return rc;
+  return LETKF_OK;
+}
+
+int letkf_gpu_get_diag(letkf_gpu_t* h, const letkf_gpu_diag* d) {
+  if (!h || !d) return LETKF_EINVAL;
+  if (!h->have_run) return fail(h, LETKF_ESTATE, "no analysis: call letkf_gpu_run first");
+  const int64_t nij = h->nij, G = h->G;
+  if (d->maxhz) HIPCHK(h, hipMemcpy(d->maxhz, h->d_maxhz, 4 * nij, hipMemcpyDeviceToHost));
+  std::vector<int32_t> ib;
+  if (d->obscount) {
+    ib.resize((size_t)nij);
+    HIPCHK(h, hipMemcpy(ib.data(), h->d_obscount, 4 * nij, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nij; ++i) d->obscount[i] = (float)ib[i];
+  }
+  // device layout (G, nij) -> reference layout (nij, G)
+  if (d->lg_obscount) {
+    ib.resize((size_t)(nij * G));
+    HIPCHK(h, hipMemcpy(ib.data(), h->d_lgm, 4 * nij * G, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nij; ++i)
+      for (int64_t g = 0; g < G; ++g) d->lg_obscount[i + nij * g] = (float)ib[i * G + g];
+  }
+  if (d->lg_obsloc) {
+    std::vector<double> db((size_t)(nij * G));
+    HIPCHK(h, hipMemcpy(db.data(), h->d_lgloc, 8 * nij * G, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nij; ++i)
+      for (int64_t g = 0; g < G; ++g) d->lg_obsloc[i + nij * g] = (float)db[i * G + g];
+  }
+  if (d->lg_selhash) {
+    std::vector<uint64_t> ub((size_t)(nij * G));
+    HIPCHK(h, hipMemcpy(ub.data(), h->d_lghash, 8 * nij * G, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nij; ++i)
+      for (int64_t g = 0; g < G; ++g) d->lg_selhash[i + nij * g] = ub[i * G + g];
+  }
+  return LETKF_OK;
+}
+
+int letkf_gpu_get_selection(letkf_gpu_t* h, int64_t ij, int32_t group, int32_t* idx_out,
+                            float* rloc_out, int64_t cap, int64_t* m_out) {
+  if (!h || !m_out) return LETKF_EINVAL;
+  if (!h->have_run) return fail(h, LETKF_ESTATE, "no analysis: call letkf_gpu_run first");
+  if (ij < 0 || ij >= h->nij || group < 0 || group >= h->G)
+    return fail(h, LETKF_EINVAL, "gridpoint / group out of range");
+  // re-run the search of the chunk that holds ij
+  const int64_t i0 = ij;
+  ChunkDev C{i0, 1, h->G, h->d_cnt, h->d_off, h->d_solve, h->d_list_idx, h->d_list_w};
+  StateDev S{h->k, h->ns, h->nij, h->d_state, h->d_mean, h->d_sprd, h->d_lat, h->d_lon,
+             h->d_mask, h->d_gx, h->d_gy, h->d_gz};
+  ObsDev O{h->nobs, h->d_ox, h->d_oy, h->d_oz, h->d_ozdim, h->d_oerr, h->d_odep,
+           h->d_oidx, h->d_osurf, h->d_otwp, h->d_otws, h->d_ohx, h->d_cell_start};
+  VarsDev V{h->nvars, h->d_slab_var, h->d_slab_lev, h->d_inc_max, h->d_bounds, h->d_lev_depth};
+  DiagDev D{h->d_maxhz, h->d_obscount, h->d_lgm, h->d_lgloc, h->d_lghash, h->d_mld,
+            h->d_maxcount, h->d_stats};
+  hipStream_t st = h->stream;
+  launch_search(st, false, h->loc, O, S, V, C, D);
+  size_t tb = h->tmp_bytes;
+  HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_tmp, tb, h->d_cnt, h->d_off, (int)h->G, st));
+  HIPCHK(h, hipMemcpyAsync(h->h_tot, h->d_off + group, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipMemcpyAsync(h->h_tot + 1, h->d_cnt + group, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipMemcpyAsync(h->h_tot + 2, h->d_off + (h->G - 1), 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipMemcpyAsync(h->h_tot + 3, h->d_cnt + (h->G - 1), 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  const int64_t off = h->h_tot[0], m = h->h_tot[1], total = h->h_tot[2] + h->h_tot[3];
+  if (total > h->list_cap) {
+    const int64_t c2 = total + 1024;
+    HIPCHK(h, dalloc(&h->d_list_idx, (size_t)c2));
+    HIPCHK(h, dalloc(&h->d_list_w, (size_t)c2));
+    h->list_cap = c2;
+    C.list_idx = h->d_list_idx;
+    C.list_w = h->d_list_w;
+  }
+  launch_search(st, true, h->loc, O, S, V, C, D);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(st));
+  *m_out = m;
+  const int64_t nc = std::min(m, cap);
+  if (nc > 0) {
+    std::vector<int32_t> sidx((size_t)nc);
+    HIPCHK(h, hipMemcpy(sidx.data(), h->d_list_idx + off, 4 * nc, hipMemcpyDeviceToHost));
+    if (rloc_out) HIPCHK(h, hipMemcpy(rloc_out, h->d_list_w + off, 4 * nc, hipMemcpyDeviceToHost));
+    if (idx_out) {
+      std::vector<int32_t> oidx((size_t)h->nobs);
+      HIPCHK(h, hipMemcpy(oidx.data(), h->d_oidx, 4 * h->nobs, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < nc; ++i) idx_out[i] = oidx[sidx[i]];
+    }
+  }
+  return LETKF_OK;
+}
+
+int letkf_gpu_get_timing(const letkf_gpu_t* h, letkf_gpu_timing* t) {
+  if (!h || !t) return LETKF_EINVAL;
+  *t = h->timing;
+  return LETKF_OK;
+}
+
+}  // extern "C"

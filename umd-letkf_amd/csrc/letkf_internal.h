@@ -1,0 +1,139 @@
+// letkf_internal.h -- device-side descriptors shared by the kernels and the
+// host orchestration of libletkf_gpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/letkf_gpu.h"
+
+namespace lk {
+
+// search cells: 1 x 1 degree, band-major, lon cells within a band
+constexpr int kBands = 180;
+constexpr int kLonCells = 360;
+constexpr int kCells = kBands * kLonCells;
+constexpr int kTab = LETKF_MAX_INTERP + 2;
+constexpr int kMaxGroups = 128;       // largest maxgroups handled on the GPU
+constexpr int kMaxK = 128;            // largest ensemble size
+
+// completed linearinterp_lat tables and the localizer switches
+struct LocDev {
+  int cls;
+  int maxgroups;
+  int n_hz, n_prof, n_sat;
+  float hz_lat[kTab], hz_dist[kTab];
+  float prof_lat[kTab], prof_dist[kTab];
+  float sat_lat[kTab], sat_dist[kTab];
+  float tloc_prof, tloc_sat;
+  int vtloc_surf;
+  float bkg_t_delta;
+  int bkg_t_var_idx, bkg_t_var_len;
+  int max_var_lev;                    // largest levels over variables
+  float vt_scale;
+  int nlev;
+};
+
+// sorted observations (search order) and their per-localizer factors
+struct ObsDev {
+  int64_t n;
+  const double* x; const double* y; const double* z;
+  const float* zdim;
+  const float* err;
+  const float* dep;          // val - hx_mean in FP32
+  const int32_t* oidx;       // original index
+  const uint8_t* surf;       // LOC_OCEAN surface flag
+  const float* tw_prof;      // GC(time, tloc_prof) or 1
+  const float* tw_sat;
+  const float* hx;           // (k, n) perturbations, sorted columns
+  const int32_t* cell_start; // kCells + 1
+};
+
+// background state
+struct StateDev {
+  int64_t k, ns, nij;
+  const float* state;  // (k, ns, nij)
+  const float* mean;   // (ns, nij)
+  const float* sprd;
+  const float* lat;
+  const float* lon;
+  const int32_t* mask;
+  const double* gx; const double* gy; const double* gz;
+};
+
+struct VarsDev {
+  int nvars;
+  const int32_t* slab_var;   // (ns) variable of each slab, -1 if none
+  const int32_t* slab_lev;   // (ns) 0-based level within the variable
+  const float* inc_max;      // (nvars)
+  const float* bounds;       // (2, nvars)
+  const float* lev_depth;    // LOC_VGAUSS level depths
+};
+
+// per-chunk work arrays; pair = (ij - i0) * G + g
+struct ChunkDev {
+  int64_t i0, ngp;
+  int G;
+  int64_t* cnt;        // kept count per pair
+  int64_t* off;        // list offset per pair
+  uint8_t* solve;      // pair needs a transform
+  int32_t* list_idx;   // sorted obs index
+  float* list_w;       // localization weight
+};
+
+struct DiagDev {
+  float* maxhz;        // (nij)
+  int32_t* obscount;   // (nij)
+  int32_t* lg_m;       // (G, nij)
+  double* lg_obsloc;   // (G, nij)
+  uint64_t* lg_hash;   // (G, nij)
+  int32_t* mld;        // (nij) mixed-layer level (LOC_OCEAN bkg_t)
+  int32_t* max_count;  // largest search set (capacity check)
+  unsigned long long* stats;  // [0] solves, [1] candidates scanned
+};
+
+// group of slab s at a gridpoint (letkf_loc_ocean.F90:360-375 for bkg_t)
+__device__ __forceinline__ int slab_group(const LocDev& L, const VarsDev& V,
+                                          int s, int mld) {
+  if (L.cls == LETKF_LOC_VGAUSS) return V.slab_lev[s];
+  if (L.cls == LETKF_LOC_OCEAN && L.vtloc_surf == 1) {
+    if (V.slab_var[s] < 0) return -1;
+    return (V.slab_lev[s] + 1 <= mld) ? 0 : 1;
+  }
+  return 0;
+}
+
+// host launchers (kernels_*.hip)
+void launch_obs_prep(hipStream_t st, int64_t n, int k, const float* lat,
+                     const float* lon, int32_t* cell, int32_t* iota);
+void launch_obs_gather(hipStream_t st, int64_t n, int k, const int32_t* perm,
+                       const float* lat, const float* lon, const float* zdim,
+                       const float* time, const float* val, const float* err,
+                       const int32_t* obsid, const int32_t* platid,
+                       const float* hx, const float* hxmean,
+                       const int32_t* sorted_cell, double* x, double* y,
+                       double* z, float* ozdim, float* oerr, float* odep,
+                       int32_t* oidx, int32_t* oobsid, int32_t* oplatid,
+                       float* otime, float* ohx, int32_t* cell_start);
+void launch_obs_flags(hipStream_t st, const LocDev& L, int64_t n,
+                      const int32_t* obsid, const int32_t* platid,
+                      const float* time, const int32_t* sat_obs, int n_sat_obs,
+                      const int32_t* sat_plats, int n_sat_plats, uint8_t* surf,
+                      float* tw_prof, float* tw_sat);
+void launch_grid_xyz(hipStream_t st, int64_t n, const float* lat,
+                     const float* lon, double* x, double* y, double* z);
+void launch_search(hipStream_t st, bool fill, const LocDev& L,
+                   const ObsDev& O, const StateDev& S, const VarsDev& V,
+                   const ChunkDev& C, const DiagDev& D);
+void launch_product_f64(hipStream_t st, const ObsDev& O, const StateDev& S,
+                        const ChunkDev& C, double* B, double* bvec);
+void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
+                            const VarsDev& V, const ChunkDev& C,
+                            const DiagDev& D, const double* B,
+                            const double* bvec, double* ana);
+void launch_finalize(hipStream_t st, const LocDev& L, const StateDev& S,
+                     const VarsDev& V, const ChunkDev& C, const DiagDev& D,
+                     float rtps, float rtpp, float mul, const double* ana_in,
+                     double* ana, double* amean, double* asprd);
+
+}  // namespace lk

@@ -76,7 +76,8 @@ class Timing(C.Structure):
     _fields_ = [("search_ms", C.c_double), ("product_ms", C.c_double),
                 ("eigen_ms", C.c_double), ("finalize_ms", C.c_double),
                 ("total_ms", C.c_double), ("n_problems", C.c_int64),
-                ("n_pairs", C.c_int64), ("n_cand", C.c_int64)]
+                ("n_pairs", C.c_int64), ("n_cand", C.c_int64),
+                ("n_launch", C.c_int64)]
 
 
 EXPORTED = [

@@ -21,7 +21,11 @@ Configurations (sizes scale down for tests with ``scale``):
   depths, latitude-dependent radius 700 -> 200 km, vertical Gaussian 300 m
   (LOC_VGAUSS).
 * ``config3``: 0.25-degree 1440x720 surface, k=50, 2M obs on 30 swath arcs
-  with lognormal along-track density, GC 300 km.
+  with lognormal along-track density, 300 km cut-off radius (hzloc
+  L = 300 km / sqrt(40/3) = 82.2 km).  With 2M obs the mean local count is
+  N_obs x (keep-cap area) / (sphere area); a 300 km sigma-equivalent L would
+  give ~12,600 on average, against the stated 0..~5,000 range, which the
+  cut-off reading reproduces (mean ~1,000, 99th percentile ~4,300).
 * ``simple3d``: the reference's two-observation test cases
   (test/simple3d/*.yaml) on a synthetic 3-variable grid.
 """
@@ -264,8 +268,12 @@ def swath_points(g, nobs, ntracks=30, width=100e3, arc=(15000e3, 30000e3),
     return lat, lon
 
 
+# 300 km read as the cut-off (search) radius: hzloc L = 300 km / sqrt(40/3)
+C3_RADIUS = 300e3 / 3.6514837741851807
+
+
 def config3(seed=0x1E7F0003, nx=1440, ny=720, k=50, nobs=2_000_000,
-            radius=300e3, corr=500e3, sigma=0.5, sigma_o=0.25, rtps=0.6,
+            radius=C3_RADIUS, corr=500e3, sigma=0.5, sigma_o=0.25, rtps=0.6,
             **swath):
     g = rng(seed)
     lon, lat, lon_ij, lat_ij = flat_grid(nx, ny)
