@@ -99,7 +99,8 @@ __global__ __launch_bounds__(256) void k_product_f64(ObsDev O, int k, ChunkDev C
     if (bi[p] < 0) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = 16 * bi[p] + 4 * (lane >> 4) + r;
+      // v_mfma_f64_16x16x4f64: register r of lane l holds row 4r + l/16
+      const int row = 16 * bi[p] + 4 * r + (lane >> 4);
       const int col = 16 * bj[p] + (lane & 15);
       const double v = acc[p][r];
       if (row < k && col < k) {
