@@ -5,6 +5,8 @@ kept index sets, search-set sizes, radii) on every gridpoint, and analysis
 mean, spread and members within 1e-10 relative (max |gpu - oracle| /
 max |oracle|).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -53,6 +55,24 @@ def test_random_localizers(loc, k):
 
 def test_rtpp_mul_k50():
     p = synth.random_problem(seed=5, k=50, ns=2, nobs=6000, rtpp=0.7, mul=1.1)
+    _check(p)
+
+
+@pytest.mark.parametrize("k", [20, 50, 63])
+def test_jacobi_and_ql_solvers_agree(k, monkeypatch):
+    """The tridiagonal-QL pipeline and the Jacobi solver (the fallback for
+    problems the QL cannot finish) both meet the 1e-10 bar."""
+    p = synth.random_problem(seed=40 + k, k=k, ns=2, nobs=3000, rtps=0.3)
+    orc = oracle_run(p)
+    for mode in ("ql", "jacobi"):
+        monkeypatch.setenv("LETKF_GPU_EIGEN", mode)
+        gpu = run_gpu(p)
+        errs, _ = compare(p, gpu, orc)
+        print(mode, k, errs, flush=True)
+
+
+def test_k_above_pipeline_uses_jacobi():
+    p = synth.random_problem(seed=77, k=70, ns=1, nobs=1500)
     _check(p)
 
 

@@ -1,0 +1,364 @@
+// kernels_eigen.hip -- batched FP64 symmetric eigensolve and transform.
+//
+// For every (gridpoint, group) with kept observations, A = (k-1)I + Yb^T R^-1 Yb
+// (k x k, FP64, from the product kernel) is decomposed and the LETKF
+// transform is applied straight to the group's slabs
+// (src/letkf_core.F90:100-141, src/letkf_solver.F90:284-290):
+//
+//   x_a(j) = (W x_b)(j) + wbar . x_b,   W = V diag(sqrt((k-1)/lam)) V^T,
+//   wbar = V diag(1/lam) V^T (Yb^T R^-1 d)
+//
+// which equals x_b^T T with T = W + wbar 1^T, W being symmetric.
+//
+// Three kernels, each shaped for what it does:
+//  k_tridiag  one wave per problem: Householder reduction A = Q Tri Q^T
+//             (LAPACK dsytd2, lower), A in LDS, lane r owns row r;
+//  k_tql      one LANE per problem: implicit QL with Wilkinson shift on the
+//             tridiagonal (Numerical Recipes tqli), d and e held in registers
+//             through a statically unrolled, predicated sweep; every rotation
+//             (c, s) and every sweep header (l, m, count) is recorded;
+//  k_eigvec_apply  one wave per problem: replays the recorded rotations into
+//             Z (lane r owns row r, registers), back-transforms V = Q Z, then
+//             applies the transform to the group's slabs with two
+//             matrix-vector products per slab (no k x k W is formed).
+// Problems whose record space overflows or whose QL does not converge are
+// flagged and solved by the Jacobi kernel instead.
+#include <hip/hip_runtime.h>
+
+#include "letkf_internal.h"
+
+namespace lk {
+
+constexpr double kEps = 2.220446049250313e-16;
+
+// sum over the K lanes of a K-thread block (K = 32 or 64)
+template <int K>
+__device__ __forceinline__ double wsumK(double v) {
+#pragma unroll
+  for (int o = K / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, K);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Householder tridiagonalization, one K-lane block per problem (k < K <= 64)
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(64) void k_tridiag(int k, ChunkDev C, double* __restrict__ B,
+                                                const double* __restrict__ bvec,
+                                                double* __restrict__ tau,
+                                                double* __restrict__ dd,
+                                                double* __restrict__ ee) {
+  constexpr int LD = K + 1;
+  __shared__ double As[K * LD];
+  __shared__ double sv[K];
+  __shared__ double sw[K];
+  const int64_t pair = blockIdx.x;
+  if (!C.solve[pair]) return;
+  const int r = threadIdx.x;
+  double* Bp = B + pair * (int64_t)k * k;
+  for (int c = 0; c < k; ++c) {
+    if (r < k) {
+      double v = Bp[r + (int64_t)k * c];
+      if (r == c) v += (double)(k - 1);
+      As[c * LD + r] = v;
+    }
+  }
+  __syncthreads();
+  double* taup = tau + pair * K;
+  double* ep = ee + pair * K;
+  for (int j = 0; j <= k - 2; ++j) {
+    const double x = (r > j && r < k) ? As[j * LD + r] : 0.0;
+    const double alpha = As[j * LD + j + 1];
+    const double sig = wsumK<K>((r > j + 1 && r < k) ? x * x : 0.0);
+    if (sig == 0.0) {
+      if (r == 0) { taup[j] = 0.0; ep[j] = alpha; }
+      continue;
+    }
+    const double mu = sqrt(alpha * alpha + sig);
+    const double beta = alpha >= 0.0 ? -mu : mu;
+    const double t = (beta - alpha) / beta;
+    const double scal = 1.0 / (alpha - beta);
+    const double v = (r == j + 1) ? 1.0 : ((r > j + 1 && r < k) ? x * scal : 0.0);
+    sv[r] = v;
+    __syncthreads();
+    double p = 0.0;
+    for (int c = j + 1; c < k; ++c) p += As[c * LD + r] * sv[c];
+    p = (r > j && r < k) ? t * p : 0.0;
+    const double kc = 0.5 * t * wsumK<K>(p * v);
+    const double w = p - kc * v;
+    sw[r] = w;
+    __syncthreads();
+    if (r > j && r < k) {
+      for (int c = j + 1; c < k; ++c) As[c * LD + r] -= v * sw[c] + w * sv[c];
+      Bp[(int64_t)j * k + r] = v;              // reflector j, rows j+1..k-1
+    }
+    if (r == 0) { taup[j] = t; ep[j] = beta; }
+    __syncthreads();
+  }
+  if (r < K) {
+    dd[pair * K + r] = r < k ? As[r * LD + r] : 0.0;
+    if (r >= k - 1) { ep[r] = 0.0; taup[r] = 0.0; }
+  }
+  (void)bvec;
+}
+
+// ---------------------------------------------------------------------------
+// implicit QL, one lane per problem; d, e in registers (static indices)
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(64) void k_tql(int k, int64_t np, ChunkDev C,
+                                            double* __restrict__ dd,
+                                            const double* __restrict__ ee,
+                                            double2* __restrict__ rot, int cap_rot,
+                                            int4* __restrict__ itr, int cap_it,
+                                            int* __restrict__ nit_out,
+                                            int* __restrict__ status) {
+  const int64_t p = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  bool active = p < np && C.solve[p];
+  double d[K], e[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    d[i] = active ? dd[p * K + i] : 0.0;
+    e[i] = (active && i < k - 1) ? ee[p * K + i] : 0.0;
+  }
+  double2* rp = rot + (active ? p : 0) * (int64_t)cap_rot;
+  int4* hp = itr + (active ? p : 0) * (int64_t)cap_it;
+  int l = 0, iters = 0, t = 0, it = 0, st = 0;
+  const int maxit = 30 * k;
+  while (__any(active)) {
+    if (active) {
+      int m = k - 1;
+      bool found = false;
+#pragma unroll
+      for (int i = 0; i < K - 1; ++i) {
+        if (!found && i >= l && i < k - 1) {
+          const double sd = fabs(d[i]) + fabs(d[i + 1]);
+          if (fabs(e[i]) <= kEps * sd) { m = i; found = true; }
+        }
+      }
+      if (m == l) {
+        ++l;
+        if (l >= k) active = false;
+      } else if (++iters > maxit) {
+        st = 2;
+        active = false;
+      } else {
+        double dl = 0.0, dl1 = 0.0, el = 0.0, dm = 0.0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          dl = (i == l) ? d[i] : dl;
+          el = (i == l) ? e[i] : el;
+          dm = (i == m) ? d[i] : dm;
+        }
+#pragma unroll
+        for (int i = 1; i < K; ++i) dl1 = (i == l + 1) ? d[i] : dl1;
+        double g = (dl1 - dl) / (2.0 * el);
+        double r = sqrt(g * g + 1.0);
+        g = dm - dl + el / (g + (g >= 0.0 ? r : -r));
+        double s = 1.0, c = 1.0, pp = 0.0;
+        int cnt = 0;
+        bool broke = false;
+#pragma unroll
+        for (int i = K - 2; i >= 0; --i) {
+          if (!broke && i >= l && i < m) {
+            const double f = s * e[i], b = c * e[i];
+            r = sqrt(f * f + g * g);
+            e[i + 1] = r;
+            if (r == 0.0) {
+              d[i + 1] -= pp;
+              broke = true;
+            } else {
+              const double rr = 1.0 / r;
+              s = f * rr;
+              c = g * rr;
+              g = d[i + 1] - pp;
+              const double r2 = (d[i] - g) * s + 2.0 * c * b;
+              pp = s * r2;
+              d[i + 1] = g + pp;
+              g = c * r2 - b;
+              if (t < cap_rot) rp[t] = make_double2(c, s);
+              ++t;
+              ++cnt;
+            }
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          if (!broke && i == l) { d[i] -= pp; e[i] = g; }
+          if (i == m) e[i] = 0.0;
+        }
+        if (it < cap_it) hp[it] = make_int4(l, m, cnt, 0);
+        ++it;
+        if (t > cap_rot || it > cap_it) { st = 1; active = false; }
+      }
+    }
+  }
+  if (p < np && C.solve[p]) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) dd[p * K + i] = d[i];     // eigenvalues
+    nit_out[p] = it;
+    status[p] = st;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// eigenvectors (rotation replay + back-transform) and application, one
+// K-lane block per problem
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(64) void k_eigvec_apply(LocDev L, StateDev S, VarsDev V,
+                                                     ChunkDev C, DiagDev D,
+                                                     const double* __restrict__ B,
+                                                     const double* __restrict__ bvec,
+                                                     const double* __restrict__ tau,
+                                                     const double* __restrict__ lam,
+                                                     const double2* __restrict__ rot,
+                                                     int cap_rot,
+                                                     const int4* __restrict__ itr,
+                                                     int cap_it,
+                                                     const int* __restrict__ nit_in,
+                                                     const int* __restrict__ status,
+                                                     double* __restrict__ ana) {
+  constexpr int LD = K + 1;
+  __shared__ double Vs[K * LD];
+  __shared__ double2 srot[K];
+  __shared__ double sv[K], sx[K], sy[K];
+  const int64_t pair = blockIdx.x;
+  if (!C.solve[pair] || status[pair] != 0) return;
+  const int k = (int)S.k;
+  const int lane = threadIdx.x;
+
+  // Z = I, row `lane` in registers; replay the QL rotations
+  double z[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) z[c] = (c == lane) ? 1.0 : 0.0;
+  const int nit = nit_in[pair];
+  const double2* rp = rot + pair * (int64_t)cap_rot;
+  const int4* hp = itr + pair * (int64_t)cap_it;
+  int64_t t = 0;
+  for (int it = 0; it < nit; ++it) {
+    const int4 h = hp[it];
+    const int m = h.y, cnt = h.z;
+    if (lane < cnt) srot[lane] = rp[t + lane];
+    __syncthreads();
+    const int lo = m - cnt;
+#pragma unroll
+    for (int i = K - 2; i >= 0; --i) {
+      if (i < m && i >= lo) {
+        const double2 cs = srot[m - 1 - i];
+        const double f = z[i + 1];
+        z[i + 1] = cs.y * z[i] + cs.x * f;
+        z[i] = cs.x * z[i] - cs.y * f;
+      }
+    }
+    t += cnt;
+    __syncthreads();
+  }
+
+  // transpose: lane c takes column c of Z
+#pragma unroll
+  for (int c = 0; c < K; ++c) Vs[lane * LD + c] = z[c];
+  __syncthreads();
+  double zc[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) zc[r] = Vs[r * LD + lane];
+  __syncthreads();
+
+  // V = Q Z, Q = H_0 H_1 ... H_{k-2}
+  const double* Bp = B + pair * (int64_t)k * k;
+  const double* taup = tau + pair * K;
+  for (int j = k - 2; j >= 0; --j) {
+    const double tj = taup[j];
+    if (tj == 0.0) continue;
+    sv[lane] = (lane == j + 1) ? 1.0 : ((lane > j + 1 && lane < k) ? Bp[(int64_t)j * k + lane] : 0.0);
+    __syncthreads();
+    double dot = 0.0;
+#pragma unroll
+    for (int r = 0; r < K; ++r) dot += sv[r] * zc[r];
+    const double sc = tj * dot;
+#pragma unroll
+    for (int r = 0; r < K; ++r) zc[r] -= sc * sv[r];
+    __syncthreads();
+  }
+
+  // lane c holds eigenvector c; rows of V go to LDS (row-major)
+  const double lc = lane < k ? lam[pair * K + lane] : 1.0;
+  const double fc = lane < k ? sqrt((double)(k - 1) / lc) : 0.0;
+  const double ic = lane < k ? 1.0 / lc : 0.0;
+#pragma unroll
+  for (int r = 0; r < K; ++r) Vs[r * LD + lane] = zc[r];
+  sx[lane] = lane < k ? bvec[pair * (int64_t)k + lane] : 0.0;
+  __syncthreads();
+  // wbar = V diag(1/lam) V^T b
+  double u = 0.0;
+#pragma unroll
+  for (int r = 0; r < K; ++r) u += zc[r] * sx[r];
+  sy[lane] = u * ic;
+  __syncthreads();
+  double wbar = 0.0;
+#pragma unroll 8
+  for (int c = 0; c < K; ++c) wbar += Vs[lane * LD + c] * sy[c];
+  __syncthreads();
+
+  const int64_t t_gp = pair / C.G;
+  const int g = (int)(pair - t_gp * C.G);
+  const int64_t ij = C.i0 + t_gp;
+  const int mld = D.mld[ij];
+  const float* xb0 = S.state + (int64_t)S.k * S.ns * ij;
+  double* xa0 = ana + (int64_t)S.k * S.ns * ij;
+  for (int s = 0; s < S.ns; ++s) {
+    if (slab_group(L, V, s, mld) != g) continue;
+    const double xr = lane < k ? (double)xb0[(int64_t)k * s + lane] : 0.0;
+    sx[lane] = xr;
+    const double wx = wsumK<K>(wbar * xr);
+    __syncthreads();
+    double y = 0.0;
+#pragma unroll
+    for (int r = 0; r < K; ++r) y += zc[r] * sx[r];
+    sy[lane] = y * fc;
+    __syncthreads();
+    double o = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < K; ++c) o += Vs[lane * LD + c] * sy[c];
+    if (lane < k) xa0[(int64_t)k * s + lane] = o + wx;
+    __syncthreads();
+  }
+}
+
+struct EigenBufs {
+  double* tau; double* d; double* e; double2* rot; int4* itr; int* nit; int* status;
+  int cap_rot, cap_it;
+};
+
+int eigen_pipeline_K(int k) { return k <= 31 ? 32 : (k <= 63 ? 64 : 0); }
+
+void eigen_caps(int k, int* cap_rot, int* cap_it) {
+  const int K = eigen_pipeline_K(k);
+  *cap_rot = 2 * K * K;
+  *cap_it = 8 * K;
+}
+
+void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
+                           const VarsDev& V, const ChunkDev& C, const DiagDev& D,
+                           double* B, const double* bvec, double* tau, double* dd,
+                           double* ee, double2* rot, int cap_rot, int4* itr,
+                           int cap_it, int* nit, int* status, double* ana) {
+  const int64_t np = C.ngp * C.G;
+  if (np <= 0) return;
+  const int k = (int)S.k;
+  const unsigned nb = (unsigned)np;
+  const unsigned nl = (unsigned)((np + 63) / 64);
+  if (eigen_pipeline_K(k) == 32) {
+    k_tridiag<32><<<nb, 32, 0, st>>>(k, C, B, bvec, tau, dd, ee);
+    k_tql<32><<<nl, 64, 0, st>>>(k, np, C, dd, ee, rot, cap_rot, itr, cap_it, nit, status);
+    k_eigvec_apply<32><<<nb, 32, 0, st>>>(L, S, V, C, D, B, bvec, tau, dd, rot, cap_rot,
+                                          itr, cap_it, nit, status, ana);
+  } else {
+    k_tridiag<64><<<nb, 64, 0, st>>>(k, C, B, bvec, tau, dd, ee);
+    k_tql<64><<<nl, 64, 0, st>>>(k, np, C, dd, ee, rot, cap_rot, itr, cap_it, nit, status);
+    k_eigvec_apply<64><<<nb, 64, 0, st>>>(L, S, V, C, D, B, bvec, tau, dd, rot, cap_rot,
+                                          itr, cap_it, nit, status, ana);
+  }
+}
+
+}  // namespace lk

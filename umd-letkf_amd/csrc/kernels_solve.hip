@@ -152,6 +152,7 @@ __global__ __launch_bounds__(256) void k_eigen_apply(LocDev L, StateDev S,
                                                      DiagDev D,
                                                      const double* __restrict__ B,
                                                      const double* __restrict__ bvec,
+                                                     const int* __restrict__ only_flagged,
                                                      double* __restrict__ ana) {
   extern __shared__ double lds[];
   const int k = (int)S.k;
@@ -170,6 +171,7 @@ __global__ __launch_bounds__(256) void k_eigen_apply(LocDev L, StateDev S,
 
   const int64_t pair = blockIdx.x;
   if (!C.solve[pair]) return;
+  if (only_flagged && only_flagged[pair] == 0) return;
   const int tid = threadIdx.x;
   const int64_t t = pair / C.G;
   const int g = (int)(pair - t * C.G);
@@ -179,7 +181,8 @@ __global__ __launch_bounds__(256) void k_eigen_apply(LocDev L, StateDev S,
   for (int e = tid; e < n * n; e += 256) {
     const int i = e / n, j = e - i * n;
     double v = 0.0;
-    if (i < k && j < k) v = Bp[i + (int64_t)k * j];
+    // upper triangle only: the tridiagonalization overwrites the lower one
+    if (i < k && j < k) v = i <= j ? Bp[i + (int64_t)k * j] : Bp[j + (int64_t)k * i];
     if (i == j) v += (i < k) ? (double)(k - 1) : 1.0;   // dummy player: isolated 1
     A[i * lda + j] = v;
     Vm[i * lda + j] = (i == j) ? 1.0 : 0.0;
@@ -297,7 +300,7 @@ size_t eigen_lds_bytes(int k) {
 void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
                             const VarsDev& V, const ChunkDev& C,
                             const DiagDev& D, const double* B,
-                            const double* bvec, double* ana) {
+                            const double* bvec, const int* only_flagged, double* ana) {
   const int64_t np = C.ngp * C.G;
   if (np <= 0) return;
   const size_t lds = eigen_lds_bytes((int)S.k);
@@ -307,7 +310,7 @@ void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
     attr = true;
   }
-  k_eigen_apply<<<(unsigned)np, 256, lds, st>>>(L, S, V, C, D, B, bvec, ana);
+  k_eigen_apply<<<(unsigned)np, 256, lds, st>>>(L, S, V, C, D, B, bvec, only_flagged, ana);
 }
 
 // ---------------------------------------------------------------------------

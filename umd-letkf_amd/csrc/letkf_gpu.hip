@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -80,6 +81,12 @@ struct letkf_gpu_t {
   float* d_list_w = nullptr;
   int64_t list_cap = 0;
   double *d_B = nullptr, *d_b = nullptr;
+  // eigensolver work (per chunk pair)
+  double *d_tau = nullptr, *d_dd = nullptr, *d_ee = nullptr;
+  double2* d_rot = nullptr;
+  int4* d_itr = nullptr;
+  int *d_nit = nullptr, *d_status = nullptr;
+  int cap_rot = 0, cap_it = 0, eigK = 0;
   void* d_tmp = nullptr;
   size_t tmp_bytes = 0;
   int64_t* h_tot = nullptr;    // pinned
@@ -183,6 +190,8 @@ int letkf_gpu_destroy(letkf_gpu_t* h) {
   dfree(h->d_lgloc); dfree(h->d_lghash);
   dfree(h->d_cnt); dfree(h->d_off); dfree(h->d_solve); dfree(h->d_list_idx);
   dfree(h->d_list_w); dfree(h->d_B); dfree(h->d_b); dfree(h->d_tmp);
+  dfree(h->d_tau); dfree(h->d_dd); dfree(h->d_ee); dfree(h->d_rot); dfree(h->d_itr);
+  dfree(h->d_nit); dfree(h->d_status);
   if (h->h_tot) hipHostFree(h->h_tot);
   hipStreamDestroy(h->stream);
   delete h;
@@ -485,8 +494,16 @@ static int ensure_outputs(letkf_gpu_t* h) {
   HIPCHK(h, dalloc(&h->d_lgm, (size_t)(G * nij)));
   HIPCHK(h, dalloc(&h->d_lgloc, (size_t)(G * nij)));
   HIPCHK(h, dalloc(&h->d_lghash, (size_t)(G * nij)));
-  // chunk sizing: bound the per-pair FP64 matrices to about 4 GB
-  int64_t pairs = std::max<int64_t>(1, std::min<int64_t>(131072, (4ll << 30) / (8 * k * k)));
+  // chunk sizing: bound the per-pair work arrays to about 12 GB
+  // LETKF_GPU_EIGEN=jacobi selects the parallel Jacobi solver for every
+  // problem (reference solver for tests); default: tridiagonal QL pipeline
+  const char* ev = getenv("LETKF_GPU_EIGEN");
+  const bool force_jacobi = ev && strcmp(ev, "jacobi") == 0;
+  h->eigK = force_jacobi ? 0 : eigen_pipeline_K((int)k);
+  if (h->eigK) eigen_caps((int)k, &h->cap_rot, &h->cap_it);
+  const int64_t per_pair = 8 * k * k + 8 * k + 3 * 8 * (int64_t)h->eigK +
+                           16 * (int64_t)h->cap_rot + 16 * (int64_t)h->cap_it + 32;
+  int64_t pairs = std::max<int64_t>(1, std::min<int64_t>(131072, (12ll << 30) / per_pair));
   int64_t cg = std::max<int64_t>(1, pairs / G);
   cg = std::min<int64_t>(cg, std::max<int64_t>(nij, 1));
   h->chunk_gp = cg;
@@ -496,6 +513,15 @@ static int ensure_outputs(letkf_gpu_t* h) {
   HIPCHK(h, dalloc(&h->d_solve, (size_t)np));
   HIPCHK(h, dalloc(&h->d_B, (size_t)(np * k * k)));
   HIPCHK(h, dalloc(&h->d_b, (size_t)(np * k)));
+  HIPCHK(h, dalloc(&h->d_status, (size_t)np));
+  if (h->eigK) {
+    HIPCHK(h, dalloc(&h->d_tau, (size_t)(np * h->eigK)));
+    HIPCHK(h, dalloc(&h->d_dd, (size_t)(np * h->eigK)));
+    HIPCHK(h, dalloc(&h->d_ee, (size_t)(np * h->eigK)));
+    HIPCHK(h, dalloc(&h->d_rot, (size_t)(np * h->cap_rot)));
+    HIPCHK(h, dalloc(&h->d_itr, (size_t)(np * h->cap_it)));
+    HIPCHK(h, dalloc(&h->d_nit, (size_t)np));
+  }
   size_t tb = 0;
   HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, h->d_cnt, h->d_off, (int)np, h->stream));
   HIPCHK(h, dalloc((char**)&h->d_tmp, tb));
@@ -577,7 +603,16 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     launch_product_f64(st, O, S, C, h->d_B, h->d_b);
     HIPCHK(h, hipGetLastError());
     rec();
-    launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_ana);
+    if (h->eigK) {
+      launch_eigen_pipeline(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_tau, h->d_dd,
+                            h->d_ee, h->d_rot, h->cap_rot, h->d_itr, h->cap_it, h->d_nit,
+                            h->d_status, h->d_ana);
+      HIPCHK(h, hipGetLastError());
+      // problems the QL could not finish go to the Jacobi solver
+      launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_status, h->d_ana);
+    } else {
+      launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, nullptr, h->d_ana);
+    }
     HIPCHK(h, hipGetLastError());
     rec();
     launch_finalize(st, h->loc, S, V, C, D, h->rtps, h->rtpp, h->mul, h->d_ana, h->d_ana,

@@ -130,7 +130,14 @@ void launch_product_f64(hipStream_t st, const ObsDev& O, const StateDev& S,
 void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
                             const VarsDev& V, const ChunkDev& C,
                             const DiagDev& D, const double* B,
-                            const double* bvec, double* ana);
+                            const double* bvec, const int* only_flagged, double* ana);
+int eigen_pipeline_K(int k);
+void eigen_caps(int k, int* cap_rot, int* cap_it);
+void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
+                           const VarsDev& V, const ChunkDev& C, const DiagDev& D,
+                           double* B, const double* bvec, double* tau, double* dd,
+                           double* ee, double2* rot, int cap_rot, int4* itr,
+                           int cap_it, int* nit, int* status, double* ana);
 void launch_finalize(hipStream_t st, const LocDev& L, const StateDev& S,
                      const VarsDev& V, const ChunkDev& C, const DiagDev& D,
                      float rtps, float rtpp, float mul, const double* ana_in,
