@@ -17,10 +17,12 @@
 //             tridiagonal (Numerical Recipes tqli), d and e held in registers
 //             through a statically unrolled, predicated sweep; every rotation
 //             (c, s) and every sweep header (l, m, count) is recorded;
-//  k_eigvec_apply  one wave per problem: replays the recorded rotations into
-//             Z (lane r owns row r, registers), back-transforms V = Q Z, then
-//             applies the transform to the group's slabs with two
-//             matrix-vector products per slab (no k x k W is formed).
+//  k_replay   one wave per problem: replays the recorded rotations into Z
+//             (lane r owns row r, registers; records prefetched two sweeps
+//             ahead and broadcast with readlane);
+//  k_vec_apply  one wave per problem: back-transforms V = Q Z and applies
+//             the transform to the group's slabs with two matrix-vector
+//             products per slab (no k x k W is formed).
 // Problems whose record space overflows or whose QL does not converge are
 // flagged and solved by the Jacobi kernel instead.
 #include <hip/hip_runtime.h>
@@ -162,15 +164,19 @@ __global__ __launch_bounds__(64) void k_tql(int k, int64_t np, ChunkDev C,
         for (int i = K - 2; i >= 0; --i) {
           if (!broke && i >= l && i < m) {
             const double f = s * e[i], b = c * e[i];
-            r = sqrt(f * f + g * g);
-            e[i + 1] = r;
-            if (r == 0.0) {
+            const double r2s = f * f + g * g;
+            if (r2s == 0.0) {
+              e[i + 1] = 0.0;
               d[i + 1] -= pp;
               broke = true;
             } else {
-              const double rr = 1.0 / r;
-              s = f * rr;
-              c = g * rr;
+              // 1/sqrt(f^2+g^2): hardware estimate + two Newton steps
+              double y = __builtin_amdgcn_rsq(r2s);
+              y = y * (1.5 - 0.5 * r2s * y * y);
+              y = y * (1.5 - 0.5 * r2s * y * y);
+              e[i + 1] = r2s * y;
+              s = f * y;
+              c = g * y;
               g = d[i + 1] - pp;
               const double r2 = (d[i] - g) * s + 2.0 * c * b;
               pp = s * r2;
@@ -202,94 +208,125 @@ __global__ __launch_bounds__(64) void k_tql(int k, int64_t np, ChunkDev C,
 }
 
 // ---------------------------------------------------------------------------
-// eigenvectors (rotation replay + back-transform) and application, one
-// K-lane block per problem
+// rotation replay: Z = product of the recorded QL rotations, one K-lane
+// block per problem, lane r owns row r of Z in registers.  Records of the
+// next two sweeps are prefetched into registers (one record per lane) and
+// broadcast with readlane, so the replay never waits on memory or LDS.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double bcast(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
 template <int K>
-__global__ __launch_bounds__(64) void k_eigvec_apply(LocDev L, StateDev S, VarsDev V,
-                                                     ChunkDev C, DiagDev D,
-                                                     const double* __restrict__ B,
-                                                     const double* __restrict__ bvec,
-                                                     const double* __restrict__ tau,
-                                                     const double* __restrict__ lam,
-                                                     const double2* __restrict__ rot,
-                                                     int cap_rot,
-                                                     const int4* __restrict__ itr,
-                                                     int cap_it,
-                                                     const int* __restrict__ nit_in,
-                                                     const int* __restrict__ status,
-                                                     double* __restrict__ ana) {
-  constexpr int LD = K + 1;
-  __shared__ double Vs[K * LD];
-  __shared__ double2 srot[K];
-  __shared__ double sv[K], sx[K], sy[K];
+__global__ __launch_bounds__(64) void k_replay(ChunkDev C, const double2* __restrict__ rot,
+                                               int cap_rot, const int4* __restrict__ itr,
+                                               int cap_it, const int* __restrict__ nit_in,
+                                               const int* __restrict__ status,
+                                               double* __restrict__ Zout) {
+  __shared__ int2 sh[8 * K];
   const int64_t pair = blockIdx.x;
   if (!C.solve[pair] || status[pair] != 0) return;
-  const int k = (int)S.k;
   const int lane = threadIdx.x;
-
-  // Z = I, row `lane` in registers; replay the QL rotations
-  double z[K];
-#pragma unroll
-  for (int c = 0; c < K; ++c) z[c] = (c == lane) ? 1.0 : 0.0;
   const int nit = nit_in[pair];
   const double2* rp = rot + pair * (int64_t)cap_rot;
   const int4* hp = itr + pair * (int64_t)cap_it;
+  for (int q = lane; q < nit; q += K) {
+    const int4 h = hp[q];
+    sh[q] = make_int2(h.y, h.z);               // (m, count)
+  }
+  __syncthreads();
+  double z[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) z[c] = (c == lane) ? 1.0 : 0.0;
+  auto fetch = [&](int64_t t0) -> double2 {
+    const int64_t q = t0 + lane;
+    return q < cap_rot ? rp[q] : make_double2(0.0, 0.0);
+  };
   int64_t t = 0;
+  double2 cs0 = fetch(0);
+  int64_t t1 = nit > 0 ? sh[0].y : 0;
+  double2 cs1 = fetch(t1);
   for (int it = 0; it < nit; ++it) {
-    const int4 h = hp[it];
-    const int m = h.y, cnt = h.z;
-    if (lane < cnt) srot[lane] = rp[t + lane];
-    __syncthreads();
+    const int m = sh[it].x, cnt = sh[it].y;
+    const int64_t t2 = t1 + (it + 1 < nit ? sh[it + 1].y : 0);
+    const double2 cs2 = fetch(t2);
     const int lo = m - cnt;
 #pragma unroll
     for (int i = K - 2; i >= 0; --i) {
       if (i < m && i >= lo) {
-        const double2 cs = srot[m - 1 - i];
+        const int q = m - 1 - i;
+        const double c = bcast(cs0.x, q), sn = bcast(cs0.y, q);
         const double f = z[i + 1];
-        z[i + 1] = cs.y * z[i] + cs.x * f;
-        z[i] = cs.x * z[i] - cs.y * f;
+        z[i + 1] = sn * z[i] + c * f;
+        z[i] = c * z[i] - sn * f;
       }
     }
     t += cnt;
-    __syncthreads();
+    cs0 = cs1;
+    cs1 = cs2;
+    t1 = t2;
   }
-
-  // transpose: lane c takes column c of Z
+  double* Zp = Zout + pair * (int64_t)K * K;
 #pragma unroll
-  for (int c = 0; c < K; ++c) Vs[lane * LD + c] = z[c];
-  __syncthreads();
+  for (int c = 0; c < K; ++c) Zp[lane * K + c] = z[c];
+}
+
+// ---------------------------------------------------------------------------
+// V = Q Z (reflectors bulk-loaded into LDS) and the application to the
+// group's slabs; lane c owns column c (eigenvector c)
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(64) void k_vec_apply(LocDev L, StateDev S, VarsDev V,
+                                                  ChunkDev C, DiagDev D,
+                                                  const double* __restrict__ B,
+                                                  const double* __restrict__ bvec,
+                                                  const double* __restrict__ tau,
+                                                  const double* __restrict__ lam,
+                                                  const double* __restrict__ Zin,
+                                                  const int* __restrict__ status,
+                                                  double* __restrict__ ana) {
+  constexpr int LD = K + 1;
+  __shared__ double Hs[K * LD];      // reflectors, then rows of V
+  __shared__ double st[K], sx[K], sy[K];
+  const int64_t pair = blockIdx.x;
+  if (!C.solve[pair] || status[pair] != 0) return;
+  const int k = (int)S.k;
+  const int lane = threadIdx.x;
+  const double* Bp = B + pair * (int64_t)k * k;
+  // reflector j occupies rows j+1..k-1 of column j (row j+1 is an implicit 1)
+  for (int e = lane; e < K * K; e += K) {
+    const int j = e / K, r = e - j * K;
+    Hs[j * LD + r] = (r == j + 1) ? 1.0 : ((r > j + 1 && r < k && j <= k - 2) ? Bp[j * k + r] : 0.0);
+  }
+  st[lane] = tau[pair * K + lane];
+  const double* Zp = Zin + pair * (int64_t)K * K;
   double zc[K];
 #pragma unroll
-  for (int r = 0; r < K; ++r) zc[r] = Vs[r * LD + lane];
+  for (int r = 0; r < K; ++r) zc[r] = Zp[r * K + lane];
   __syncthreads();
-
-  // V = Q Z, Q = H_0 H_1 ... H_{k-2}
-  const double* Bp = B + pair * (int64_t)k * k;
-  const double* taup = tau + pair * K;
   for (int j = k - 2; j >= 0; --j) {
-    const double tj = taup[j];
+    const double tj = st[j];
     if (tj == 0.0) continue;
-    sv[lane] = (lane == j + 1) ? 1.0 : ((lane > j + 1 && lane < k) ? Bp[(int64_t)j * k + lane] : 0.0);
-    __syncthreads();
+    const double* hv = Hs + j * LD;
     double dot = 0.0;
 #pragma unroll
-    for (int r = 0; r < K; ++r) dot += sv[r] * zc[r];
+    for (int r = 0; r < K; ++r) dot += hv[r] * zc[r];
     const double sc = tj * dot;
+    // re-read the reflector from LDS rather than keeping K values live
+    asm volatile("" ::: "memory");
 #pragma unroll
-    for (int r = 0; r < K; ++r) zc[r] -= sc * sv[r];
-    __syncthreads();
+    for (int r = 0; r < K; ++r) zc[r] -= sc * hv[r];
   }
-
-  // lane c holds eigenvector c; rows of V go to LDS (row-major)
+  __syncthreads();
   const double lc = lane < k ? lam[pair * K + lane] : 1.0;
   const double fc = lane < k ? sqrt((double)(k - 1) / lc) : 0.0;
   const double ic = lane < k ? 1.0 / lc : 0.0;
 #pragma unroll
-  for (int r = 0; r < K; ++r) Vs[r * LD + lane] = zc[r];
+  for (int r = 0; r < K; ++r) Hs[r * LD + lane] = zc[r];      // rows of V
   sx[lane] = lane < k ? bvec[pair * (int64_t)k + lane] : 0.0;
   __syncthreads();
-  // wbar = V diag(1/lam) V^T b
   double u = 0.0;
 #pragma unroll
   for (int r = 0; r < K; ++r) u += zc[r] * sx[r];
@@ -297,7 +334,7 @@ __global__ __launch_bounds__(64) void k_eigvec_apply(LocDev L, StateDev S, VarsD
   __syncthreads();
   double wbar = 0.0;
 #pragma unroll 8
-  for (int c = 0; c < K; ++c) wbar += Vs[lane * LD + c] * sy[c];
+  for (int c = 0; c < K; ++c) wbar += Hs[lane * LD + c] * sy[c];
   __syncthreads();
 
   const int64_t t_gp = pair / C.G;
@@ -319,7 +356,7 @@ __global__ __launch_bounds__(64) void k_eigvec_apply(LocDev L, StateDev S, VarsD
     __syncthreads();
     double o = 0.0;
 #pragma unroll 8
-    for (int c = 0; c < K; ++c) o += Vs[lane * LD + c] * sy[c];
+    for (int c = 0; c < K; ++c) o += Hs[lane * LD + c] * sy[c];
     if (lane < k) xa0[(int64_t)k * s + lane] = o + wx;
     __syncthreads();
   }
@@ -342,7 +379,7 @@ void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
                            const VarsDev& V, const ChunkDev& C, const DiagDev& D,
                            double* B, const double* bvec, double* tau, double* dd,
                            double* ee, double2* rot, int cap_rot, int4* itr,
-                           int cap_it, int* nit, int* status, double* ana) {
+                           int cap_it, int* nit, int* status, double* Z, double* ana) {
   const int64_t np = C.ngp * C.G;
   if (np <= 0) return;
   const int k = (int)S.k;
@@ -351,13 +388,13 @@ void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
   if (eigen_pipeline_K(k) == 32) {
     k_tridiag<32><<<nb, 32, 0, st>>>(k, C, B, bvec, tau, dd, ee);
     k_tql<32><<<nl, 64, 0, st>>>(k, np, C, dd, ee, rot, cap_rot, itr, cap_it, nit, status);
-    k_eigvec_apply<32><<<nb, 32, 0, st>>>(L, S, V, C, D, B, bvec, tau, dd, rot, cap_rot,
-                                          itr, cap_it, nit, status, ana);
+    k_replay<32><<<nb, 32, 0, st>>>(C, rot, cap_rot, itr, cap_it, nit, status, Z);
+    k_vec_apply<32><<<nb, 32, 0, st>>>(L, S, V, C, D, B, bvec, tau, dd, Z, status, ana);
   } else {
     k_tridiag<64><<<nb, 64, 0, st>>>(k, C, B, bvec, tau, dd, ee);
     k_tql<64><<<nl, 64, 0, st>>>(k, np, C, dd, ee, rot, cap_rot, itr, cap_it, nit, status);
-    k_eigvec_apply<64><<<nb, 64, 0, st>>>(L, S, V, C, D, B, bvec, tau, dd, rot, cap_rot,
-                                          itr, cap_it, nit, status, ana);
+    k_replay<64><<<nb, 64, 0, st>>>(C, rot, cap_rot, itr, cap_it, nit, status, Z);
+    k_vec_apply<64><<<nb, 64, 0, st>>>(L, S, V, C, D, B, bvec, tau, dd, Z, status, ana);
   }
 }
 

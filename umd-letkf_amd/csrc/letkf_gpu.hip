@@ -86,6 +86,7 @@ struct letkf_gpu_t {
   double2* d_rot = nullptr;
   int4* d_itr = nullptr;
   int *d_nit = nullptr, *d_status = nullptr;
+  double* d_Z = nullptr;
   int cap_rot = 0, cap_it = 0, eigK = 0;
   void* d_tmp = nullptr;
   size_t tmp_bytes = 0;
@@ -191,7 +192,7 @@ int letkf_gpu_destroy(letkf_gpu_t* h) {
   dfree(h->d_cnt); dfree(h->d_off); dfree(h->d_solve); dfree(h->d_list_idx);
   dfree(h->d_list_w); dfree(h->d_B); dfree(h->d_b); dfree(h->d_tmp);
   dfree(h->d_tau); dfree(h->d_dd); dfree(h->d_ee); dfree(h->d_rot); dfree(h->d_itr);
-  dfree(h->d_nit); dfree(h->d_status);
+  dfree(h->d_nit); dfree(h->d_status); dfree(h->d_Z);
   if (h->h_tot) hipHostFree(h->h_tot);
   hipStreamDestroy(h->stream);
   delete h;
@@ -502,6 +503,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
   h->eigK = force_jacobi ? 0 : eigen_pipeline_K((int)k);
   if (h->eigK) eigen_caps((int)k, &h->cap_rot, &h->cap_it);
   const int64_t per_pair = 8 * k * k + 8 * k + 3 * 8 * (int64_t)h->eigK +
+                           8 * (int64_t)h->eigK * h->eigK +
                            16 * (int64_t)h->cap_rot + 16 * (int64_t)h->cap_it + 32;
   int64_t pairs = std::max<int64_t>(1, std::min<int64_t>(131072, (12ll << 30) / per_pair));
   int64_t cg = std::max<int64_t>(1, pairs / G);
@@ -521,6 +523,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
     HIPCHK(h, dalloc(&h->d_rot, (size_t)(np * h->cap_rot)));
     HIPCHK(h, dalloc(&h->d_itr, (size_t)(np * h->cap_it)));
     HIPCHK(h, dalloc(&h->d_nit, (size_t)np));
+    HIPCHK(h, dalloc(&h->d_Z, (size_t)(np * h->eigK * h->eigK)));
   }
   size_t tb = 0;
   HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, h->d_cnt, h->d_off, (int)np, h->stream));
@@ -606,7 +609,7 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     if (h->eigK) {
       launch_eigen_pipeline(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_tau, h->d_dd,
                             h->d_ee, h->d_rot, h->cap_rot, h->d_itr, h->cap_it, h->d_nit,
-                            h->d_status, h->d_ana);
+                            h->d_status, h->d_Z, h->d_ana);
       HIPCHK(h, hipGetLastError());
       // problems the QL could not finish go to the Jacobi solver
       launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_status, h->d_ana);

@@ -137,7 +137,7 @@ void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
                            const VarsDev& V, const ChunkDev& C, const DiagDev& D,
                            double* B, const double* bvec, double* tau, double* dd,
                            double* ee, double2* rot, int cap_rot, int4* itr,
-                           int cap_it, int* nit, int* status, double* ana);
+                           int cap_it, int* nit, int* status, double* Z, double* ana);
 void launch_finalize(hipStream_t st, const LocDev& L, const StateDev& S,
                      const VarsDev& V, const ChunkDev& C, const DiagDev& D,
                      float rtps, float rtpp, float mul, const double* ana_in,
