@@ -72,8 +72,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--path", default="plain", choices=["plain", "batched"])
-    ap.add_argument("--precision", default="fp64", choices=list(PEAK))
+    ap.add_argument("--path", default="batched", choices=["plain", "batched"])
+    ap.add_argument("--precision", default="bf16x3", choices=list(PEAK))
+    ap.add_argument("--no-fp64-compare", action="store_true",
+                    help="skip timing the plain FP64 path on the same workload")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -161,6 +163,22 @@ def main():
             "mean_maxabs": float(np.abs(mn[:, pts] - orc["mean"]).max()),
         }
     solver.final()
+    if not args.no_fp64_compare and not (args.path == "plain" and args.precision == "fp64"):
+        ref = LetkfGpuSolver(prob, precision="fp64", path="plain")
+        for _ in range(args.warmup):
+            ref.run()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ref.run()
+        dt = time.perf_counter() - t0
+        rt = ref.timing()
+        ref.final()
+        out["fp64_plain"] = {"value": round(prob.nij * args.steps / dt, 1),
+                             "ms_per_step": round(1e3 * dt / args.steps, 3),
+                             "phases_ms": {kk: round(rt[f"{kk}_ms"], 3) for kk in
+                                           ("search", "product", "eigen", "finalize")}}
+        out["speedup_vs_fp64_plain"] = round(out["value"] / out["fp64_plain"]["value"], 3)
+        log(f"[bench] fp64 plain: {out['fp64_plain']}")
     print(json.dumps(out), flush=True)
 
 

@@ -87,12 +87,21 @@ struct letkf_gpu_t {
   int4* d_itr = nullptr;
   int *d_nit = nullptr, *d_status = nullptr;
   double* d_Z = nullptr;
+  // batched path: longest-first order of the problems of a chunk
+  uint32_t *d_bkey = nullptr, *d_bkey2 = nullptr;
+  int32_t *d_bval = nullptr, *d_perm = nullptr;
+  void* d_stmp = nullptr;
+  size_t stmp_bytes = 0;
   int cap_rot = 0, cap_it = 0, eigK = 0;
   void* d_tmp = nullptr;
   size_t tmp_bytes = 0;
   int64_t* h_tot = nullptr;    // pinned
 
   letkf_gpu_timing timing{};
+
+  bool batched() const {
+    return dims.path == LETKF_PATH_BATCHED && dims.precision != LETKF_PREC_FP64;
+  }
 };
 
 static std::string g_create_err;
@@ -150,8 +159,6 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
     return fail(nullptr, LETKF_EINVAL, "invalid path %d", d->path);
   if (d->path == LETKF_PATH_PLAIN && d->precision != LETKF_PREC_FP64)
     return fail(nullptr, LETKF_EUNSUPPORTED, "the plain path computes in FP64 only");
-  if (d->path == LETKF_PATH_BATCHED)
-    return fail(nullptr, LETKF_EUNSUPPORTED, "batched path not available in this build");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(nullptr, LETKF_EHIP, "no HIP device");
@@ -193,6 +200,7 @@ int letkf_gpu_destroy(letkf_gpu_t* h) {
   dfree(h->d_list_w); dfree(h->d_B); dfree(h->d_b); dfree(h->d_tmp);
   dfree(h->d_tau); dfree(h->d_dd); dfree(h->d_ee); dfree(h->d_rot); dfree(h->d_itr);
   dfree(h->d_nit); dfree(h->d_status); dfree(h->d_Z);
+  dfree(h->d_bkey); dfree(h->d_bkey2); dfree(h->d_bval); dfree(h->d_perm); dfree(h->d_stmp);
   if (h->h_tot) hipHostFree(h->h_tot);
   hipStreamDestroy(h->stream);
   delete h;
@@ -529,6 +537,17 @@ static int ensure_outputs(letkf_gpu_t* h) {
   HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, h->d_cnt, h->d_off, (int)np, h->stream));
   HIPCHK(h, dalloc((char**)&h->d_tmp, tb));
   h->tmp_bytes = tb;
+  if (h->batched()) {
+    HIPCHK(h, dalloc(&h->d_bkey, (size_t)np));
+    HIPCHK(h, dalloc(&h->d_bkey2, (size_t)np));
+    HIPCHK(h, dalloc(&h->d_bval, (size_t)np));
+    HIPCHK(h, dalloc(&h->d_perm, (size_t)np));
+    size_t sb = 0;
+    HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(nullptr, sb, h->d_bkey, h->d_bkey2, h->d_bval,
+                                                 h->d_perm, (int)np, 0, 32, h->stream));
+    HIPCHK(h, dalloc((char**)&h->d_stmp, sb));
+    h->stmp_bytes = sb;
+  }
   h->alloc_G = G;
   return LETKF_OK;
 }
@@ -603,7 +622,15 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     launch_search(st, true, h->loc, O, S, V, C, D);
     HIPCHK(h, hipGetLastError());
     rec();
-    launch_product_f64(st, O, S, C, h->d_B, h->d_b);
+    if (h->batched()) {
+      launch_bucket_keys(st, np, h->d_cnt, h->d_solve, h->d_bkey, h->d_bval);
+      size_t sb = h->stmp_bytes;
+      HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(h->d_stmp, sb, h->d_bkey, h->d_bkey2,
+                                                   h->d_bval, h->d_perm, (int)np, 0, 32, st));
+      launch_product_mx(st, h->dims.precision, O, S, C, h->d_perm, h->d_B, h->d_b);
+    } else {
+      launch_product_f64(st, O, S, C, h->d_B, h->d_b);
+    }
     HIPCHK(h, hipGetLastError());
     rec();
     if (h->eigK) {

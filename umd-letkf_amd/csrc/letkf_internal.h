@@ -131,6 +131,10 @@ void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
                             const VarsDev& V, const ChunkDev& C,
                             const DiagDev& D, const double* B,
                             const double* bvec, const int* only_flagged, double* ana);
+void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev& S,
+                       const ChunkDev& C, const int32_t* perm, double* B, double* bvec);
+void launch_bucket_keys(hipStream_t st, int64_t np, const int64_t* cnt, const uint8_t* solve,
+                        uint32_t* keys, int32_t* vals);
 int eigen_pipeline_K(int k);
 void eigen_caps(int k, int* cap_rot, int* cap_it);
 void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
