@@ -19,10 +19,11 @@
 //             (c, s) and every sweep header (l, m, count) is recorded;
 //  k_replay   one wave per problem: replays the recorded rotations into Z
 //             (lane r owns row r, registers; records prefetched two sweeps
-//             ahead and broadcast with readlane);
-//  k_vec_apply  one wave per problem: back-transforms V = Q Z and applies
-//             the transform to the group's slabs with two matrix-vector
-//             products per slab (no k x k W is formed).
+//             ahead and broadcast with readlane) and stores Z;
+//  k_vec_apply  one wave per problem: back-transforms V = Q Z (reflectors
+//             bulk-loaded into LDS) and applies the transform to the group's
+//             slabs with two matrix-vector products per slab (the k x k W is
+//             never formed).
 // Problems whose record space overflows or whose QL does not converge are
 // flagged and solved by the Jacobi kernel instead.
 #include <hip/hip_runtime.h>
@@ -50,10 +51,10 @@ __global__ __launch_bounds__(64) void k_tridiag(int k, ChunkDev C, double* __res
                                                 double* __restrict__ tau,
                                                 double* __restrict__ dd,
                                                 double* __restrict__ ee) {
-  constexpr int LD = K + 1;
-  __shared__ double As[K * LD];
+  extern __shared__ double As[];      // k x (k+1), column-major
   __shared__ double sv[K];
   __shared__ double sw[K];
+  const int LD = k + 1;
   const int64_t pair = blockIdx.x;
   if (!C.solve[pair]) return;
   const int r = threadIdx.x;
@@ -83,9 +84,16 @@ __global__ __launch_bounds__(64) void k_tridiag(int k, ChunkDev C, double* __res
     const double v = (r == j + 1) ? 1.0 : ((r > j + 1 && r < k) ? x * scal : 0.0);
     sv[r] = v;
     __syncthreads();
-    double p = 0.0;
-    for (int c = j + 1; c < k; ++c) p += As[c * LD + r] * sv[c];
-    p = (r > j && r < k) ? t * p : 0.0;
+    double p0 = 0.0, p1 = 0.0;
+    if (r > j && r < k) {
+      int c = j + 1;
+      for (; c + 1 < k; c += 2) {
+        p0 += As[c * LD + r] * sv[c];
+        p1 += As[(c + 1) * LD + r] * sv[c + 1];
+      }
+      if (c < k) p0 += As[c * LD + r] * sv[c];
+    }
+    const double p = (r > j && r < k) ? t * (p0 + p1) : 0.0;
     const double kc = 0.5 * t * wsumK<K>(p * v);
     const double w = p - kc * v;
     sw[r] = w;
@@ -107,7 +115,7 @@ __global__ __launch_bounds__(64) void k_tridiag(int k, ChunkDev C, double* __res
 // ---------------------------------------------------------------------------
 // implicit QL, one lane per problem; d, e in registers (static indices)
 // ---------------------------------------------------------------------------
-template <int K>
+template <int K, int KS>
 __global__ __launch_bounds__(64) void k_tql(int k, int64_t np, ChunkDev C,
                                             double* __restrict__ dd,
                                             const double* __restrict__ ee,
@@ -120,8 +128,8 @@ __global__ __launch_bounds__(64) void k_tql(int k, int64_t np, ChunkDev C,
   double d[K], e[K];
 #pragma unroll
   for (int i = 0; i < K; ++i) {
-    d[i] = active ? dd[p * K + i] : 0.0;
-    e[i] = (active && i < k - 1) ? ee[p * K + i] : 0.0;
+    d[i] = active ? dd[p * KS + i] : 0.0;
+    e[i] = (active && i < k - 1) ? ee[p * KS + i] : 0.0;
   }
   double2* rp = rot + (active ? p : 0) * (int64_t)cap_rot;
   int4* hp = itr + (active ? p : 0) * (int64_t)cap_it;
@@ -201,17 +209,14 @@ __global__ __launch_bounds__(64) void k_tql(int k, int64_t np, ChunkDev C,
   }
   if (p < np && C.solve[p]) {
 #pragma unroll
-    for (int i = 0; i < K; ++i) dd[p * K + i] = d[i];     // eigenvalues
+    for (int i = 0; i < K; ++i) dd[p * KS + i] = d[i];    // eigenvalues
     nit_out[p] = it;
     status[p] = st;
   }
 }
 
 // ---------------------------------------------------------------------------
-// rotation replay: Z = product of the recorded QL rotations, one K-lane
-// block per problem, lane r owns row r of Z in registers.  Records of the
-// next two sweeps are prefetched into registers (one record per lane) and
-// broadcast with readlane, so the replay never waits on memory or LDS.
+// eigenvectors and application
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double bcast(double v, int lane) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
@@ -244,7 +249,6 @@ __global__ __launch_bounds__(64) void k_replay(ChunkDev C, const double2* __rest
     const int64_t q = t0 + lane;
     return q < cap_rot ? rp[q] : make_double2(0.0, 0.0);
   };
-  int64_t t = 0;
   double2 cs0 = fetch(0);
   int64_t t1 = nit > 0 ? sh[0].y : 0;
   double2 cs1 = fetch(t1);
@@ -263,7 +267,6 @@ __global__ __launch_bounds__(64) void k_replay(ChunkDev C, const double2* __rest
         z[i] = c * z[i] - sn * f;
       }
     }
-    t += cnt;
     cs0 = cs1;
     cs1 = cs2;
     t1 = t2;
@@ -273,10 +276,6 @@ __global__ __launch_bounds__(64) void k_replay(ChunkDev C, const double2* __rest
   for (int c = 0; c < K; ++c) Zp[lane * K + c] = z[c];
 }
 
-// ---------------------------------------------------------------------------
-// V = Q Z (reflectors bulk-loaded into LDS) and the application to the
-// group's slabs; lane c owns column c (eigenvector c)
-// ---------------------------------------------------------------------------
 template <int K>
 __global__ __launch_bounds__(64) void k_vec_apply(LocDev L, StateDev S, VarsDev V,
                                                   ChunkDev C, DiagDev D,
@@ -288,7 +287,7 @@ __global__ __launch_bounds__(64) void k_vec_apply(LocDev L, StateDev S, VarsDev 
                                                   const int* __restrict__ status,
                                                   double* __restrict__ ana) {
   constexpr int LD = K + 1;
-  __shared__ double Hs[K * LD];      // reflectors, then rows of V
+  __shared__ double Hs[K * LD];      // reflectors (zero padded), then rows of V
   __shared__ double st[K], sx[K], sy[K];
   const int64_t pair = blockIdx.x;
   if (!C.solve[pair] || status[pair] != 0) return;
@@ -362,11 +361,6 @@ __global__ __launch_bounds__(64) void k_vec_apply(LocDev L, StateDev S, VarsDev 
   }
 }
 
-struct EigenBufs {
-  double* tau; double* d; double* e; double2* rot; int4* itr; int* nit; int* status;
-  int cap_rot, cap_it;
-};
-
 int eigen_pipeline_K(int k) { return k <= 31 ? 32 : (k <= 63 ? 64 : 0); }
 
 void eigen_caps(int k, int* cap_rot, int* cap_it) {
@@ -375,27 +369,42 @@ void eigen_caps(int k, int* cap_rot, int* cap_it) {
   *cap_it = 8 * K;
 }
 
+template <int KQ, int K>
+static void tql_launch(hipStream_t st, unsigned nl, int k, int64_t np, const ChunkDev& C,
+                       double* dd, double* ee, double2* rot, int cap_rot, int4* itr,
+                       int cap_it, int* nit, int* status) {
+  k_tql<KQ, K><<<nl, 64, 0, st>>>(k, np, C, dd, ee, rot, cap_rot, itr, cap_it, nit, status);
+}
+
+template <int K>
+static void pipeline(hipStream_t st, const LocDev& L, const StateDev& S, const VarsDev& V,
+                     const ChunkDev& C, const DiagDev& D, double* B, const double* bvec,
+                     double* tau, double* dd, double* ee, double2* rot, int cap_rot,
+                     int4* itr, int cap_it, int* nit, int* status, double* Z,
+                     double* ana) {
+  const int64_t np = C.ngp * C.G;
+  const int k = (int)S.k;
+  const unsigned nb = (unsigned)np;
+  const unsigned nl = (unsigned)((np + 63) / 64);
+  const size_t lds = sizeof(double) * (size_t)k * (k + 1);
+  k_tridiag<K><<<nb, K, lds, st>>>(k, C, B, bvec, tau, dd, ee);
+  tql_launch<K, K>(st, nl, k, np, C, dd, ee, rot, cap_rot, itr, cap_it, nit, status);
+  k_replay<K><<<nb, K, 0, st>>>(C, rot, cap_rot, itr, cap_it, nit, status, Z);
+  k_vec_apply<K><<<nb, K, 0, st>>>(L, S, V, C, D, B, bvec, tau, dd, Z, status, ana);
+}
+
 void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
                            const VarsDev& V, const ChunkDev& C, const DiagDev& D,
                            double* B, const double* bvec, double* tau, double* dd,
                            double* ee, double2* rot, int cap_rot, int4* itr,
                            int cap_it, int* nit, int* status, double* Z, double* ana) {
-  const int64_t np = C.ngp * C.G;
-  if (np <= 0) return;
-  const int k = (int)S.k;
-  const unsigned nb = (unsigned)np;
-  const unsigned nl = (unsigned)((np + 63) / 64);
-  if (eigen_pipeline_K(k) == 32) {
-    k_tridiag<32><<<nb, 32, 0, st>>>(k, C, B, bvec, tau, dd, ee);
-    k_tql<32><<<nl, 64, 0, st>>>(k, np, C, dd, ee, rot, cap_rot, itr, cap_it, nit, status);
-    k_replay<32><<<nb, 32, 0, st>>>(C, rot, cap_rot, itr, cap_it, nit, status, Z);
-    k_vec_apply<32><<<nb, 32, 0, st>>>(L, S, V, C, D, B, bvec, tau, dd, Z, status, ana);
-  } else {
-    k_tridiag<64><<<nb, 64, 0, st>>>(k, C, B, bvec, tau, dd, ee);
-    k_tql<64><<<nl, 64, 0, st>>>(k, np, C, dd, ee, rot, cap_rot, itr, cap_it, nit, status);
-    k_replay<64><<<nb, 64, 0, st>>>(C, rot, cap_rot, itr, cap_it, nit, status, Z);
-    k_vec_apply<64><<<nb, 64, 0, st>>>(L, S, V, C, D, B, bvec, tau, dd, Z, status, ana);
-  }
+  if (C.ngp * C.G <= 0) return;
+  if (eigen_pipeline_K((int)S.k) == 32)
+    pipeline<32>(st, L, S, V, C, D, B, bvec, tau, dd, ee, rot, cap_rot, itr, cap_it, nit,
+                 status, Z, ana);
+  else
+    pipeline<64>(st, L, S, V, C, D, B, bvec, tau, dd, ee, rot, cap_rot, itr, cap_it, nit,
+                 status, Z, ana);
 }
 
 }  // namespace lk

@@ -1,14 +1,17 @@
 // kernels_product_mx.hip -- Yb^T R^-1 Yb and Yb^T R^-1 d on low-precision
 // MFMA with FP32 accumulation (the batched mixed-precision path).
 //
-// One wave per (gridpoint, group) problem.  Problems are visited in
-// decreasing order of their kept-observation count m (a permutation sorted
-// on the device), so the longest problems start first and the tail of the
-// launch is made of short ones.  The kept columns y_j (k values) are loaded
+// One wave per (gridpoint, group) problem.  Problems are visited by
+// power-of-two buckets of their kept-observation count m, largest first
+// (a permutation sorted on the device), in gridpoint order inside a bucket,
+// and blocks are remapped so that each XCD walks a contiguous range.  The kept columns y_j (k values) are loaded
 // straight from the sorted hx array into MFMA operand registers, 16
 // observations per step, the next step's loads issued before the current
-// step's MFMAs.  Row k of the padded operand carries d_j, so column k of the
-// result is Yb^T R^-1 d.  Only the upper-triangle 32x32 blocks are computed.
+// step's MFMAs.  Both MFMA operands are sqrt(rloc_j/err_j) y_j (the square
+// root written by the search's fill pass), so each value is rounded/split
+// once.  Row k of the padded operand carries the scaled d_j, so column k of
+// the result is Yb^T R^-1 d.  Only the upper-triangle 32x32 blocks are
+// computed.
 //
 //   PREC_FP32    v_mfma_f32_32x32x2_f32   (8 MFMA per block per 16 obs)
 //   PREC_BF16    v_mfma_f32_32x32x16_bf16 (1 per block per 16 obs)
@@ -26,8 +29,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 template <int NB>
 struct MxFrag {
-  float y[NB][8];   // y_j[32 rb + lane%32] for the lane's 8 observations
-  float s[8];       // R^-1 rloc of the lane's 8 observations
+  float y[NB][8];   // sqrt(rloc_j/err_j) y_j[32 rb + lane%32], the lane's 8 obs
 };
 
 // observation e (0..7) of a lane in a 16-observation step: the BF16 MFMA
@@ -48,9 +50,8 @@ __device__ __forceinline__ void load_frag(MxFrag<NB>& F, const ObsDev& O, const 
     float w = 0.0f;
     if (j < m) {
       s = C.list_idx[off + j];
-      w = C.list_w[off + j] / O.err[s];
+      w = C.list_w[off + j];            // sqrt(rloc / err), written by the search
     }
-    F.s[e] = w;
 #pragma unroll
     for (int rb = 0; rb < NB; ++rb) {
       const int a = 32 * rb + i;
@@ -59,7 +60,7 @@ __device__ __forceinline__ void load_frag(MxFrag<NB>& F, const ObsDev& O, const 
         if (a < k) v = O.hx[(int64_t)s * k + a];
         else if (a == k) v = O.dep[s];
       }
-      F.y[rb][e] = v;
+      F.y[rb][e] = v * w;
     }
   }
 }
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
                                                    double* __restrict__ bvec) {
   constexpr int NB = KP / 32;
   constexpr int NBLK = NB * (NB + 1) / 2;
-  const int64_t pair = perm[blockIdx.x];
+  const int64_t pair = perm[C.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x];
   if (!C.solve[pair]) return;
   const int64_t m = C.cnt[pair];
   const int64_t off = C.off[pair];
@@ -100,11 +101,8 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
   load_frag<NB, PREC>(cur, O, C, off, m, 0, k, lane);
   for (int64_t j0 = 0; j0 < m; j0 += 16) {
     if (j0 + 16 < m) load_frag<NB, PREC>(nxt, O, C, off, m, j0 + 16, k, lane);
-    float ya[NB][8];
-#pragma unroll
-    for (int rb = 0; rb < NB; ++rb)
-#pragma unroll
-      for (int e = 0; e < 8; ++e) ya[rb][e] = cur.y[rb][e] * cur.s[e];
+    // both operands are sqrt(R^-1 rloc) y: A B^T = Yb^T R^-1 Yb, and row k
+    // (sqrt(R^-1 rloc) d) gives Yb^T R^-1 d in column k
     if (PREC == LETKF_PREC_FP32) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
@@ -113,33 +111,30 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
         for (int rb = 0; rb < NB; ++rb)
 #pragma unroll
           for (int cb = rb; cb < NB; ++cb, ++b)
-            acc[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ya[rb][e], cur.y[cb][e], acc[b], 0, 0, 0);
+            acc[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(cur.y[rb][e], cur.y[cb][e], acc[b], 0, 0, 0);
       }
     } else if (PREC == LETKF_PREC_BF16) {
-      bf16x8 a16[NB], b16[NB];
+      bf16x8 v16[NB];
 #pragma unroll
-      for (int rb = 0; rb < NB; ++rb) { a16[rb] = to_bf16(ya[rb]); b16[rb] = to_bf16(cur.y[rb]); }
+      for (int rb = 0; rb < NB; ++rb) v16[rb] = to_bf16(cur.y[rb]);
       int b = 0;
 #pragma unroll
       for (int rb = 0; rb < NB; ++rb)
 #pragma unroll
         for (int cb = rb; cb < NB; ++cb, ++b)
-          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a16[rb], b16[cb], acc[b], 0, 0, 0);
+          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(v16[rb], v16[cb], acc[b], 0, 0, 0);
     } else {
-      bf16x8 ahi[NB], alo[NB], bhi[NB], blo[NB];
+      bf16x8 hi[NB], lo[NB];
 #pragma unroll
-      for (int rb = 0; rb < NB; ++rb) {
-        split_bf16(ya[rb], ahi[rb], alo[rb]);
-        split_bf16(cur.y[rb], bhi[rb], blo[rb]);
-      }
+      for (int rb = 0; rb < NB; ++rb) split_bf16(cur.y[rb], hi[rb], lo[rb]);
       int b = 0;
 #pragma unroll
       for (int rb = 0; rb < NB; ++rb)
 #pragma unroll
         for (int cb = rb; cb < NB; ++cb, ++b) {
-          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[rb], bhi[cb], acc[b], 0, 0, 0);
-          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[rb], blo[cb], acc[b], 0, 0, 0);
-          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[rb], bhi[cb], acc[b], 0, 0, 0);
+          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo[rb], hi[cb], acc[b], 0, 0, 0);
+          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[rb], lo[cb], acc[b], 0, 0, 0);
+          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[rb], hi[cb], acc[b], 0, 0, 0);
         }
     }
     cur = nxt;
@@ -195,21 +190,30 @@ void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev
   }
 }
 
-// keys for the longest-first order of the problems of a chunk
-__global__ void k_bucket_keys(int64_t np, const int64_t* __restrict__ cnt,
+// order of the problems of a chunk: mode 0 gridpoint order, mode 1 longest
+// first (descending m), mode 2 power-of-two buckets of m, largest first,
+// gridpoint order inside a bucket
+__global__ void k_bucket_keys(int mode, int64_t np, const int64_t* __restrict__ cnt,
                               const uint8_t* __restrict__ solve, uint32_t* keys,
                               int32_t* vals) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const int64_t m = solve[p] ? cnt[p] : 0;
-  keys[p] = 0xffffffffu - (uint32_t)(m > 0xfffffffe ? 0xfffffffe : m);
+  uint32_t key = (uint32_t)p;
+  if (mode == 1) {
+    key = 0xffffffffu - (uint32_t)(m > 0xfffffffe ? 0xfffffffe : m);
+  } else if (mode == 2) {
+    const uint32_t b = m > 0 ? 64u - (uint32_t)__clzll((unsigned long long)m) : 0u;
+    key = ((31u - (b > 31u ? 31u : b)) << 27) | (uint32_t)p;
+  }
+  keys[p] = key;
   vals[p] = (int32_t)p;
 }
 
-void launch_bucket_keys(hipStream_t st, int64_t np, const int64_t* cnt, const uint8_t* solve,
-                        uint32_t* keys, int32_t* vals) {
+void launch_bucket_keys(hipStream_t st, int mode, int64_t np, const int64_t* cnt,
+                        const uint8_t* solve, uint32_t* keys, int32_t* vals) {
   if (np <= 0) return;
-  k_bucket_keys<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(np, cnt, solve, keys, vals);
+  k_bucket_keys<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(mode, np, cnt, solve, keys, vals);
 }
 
 }  // namespace lk

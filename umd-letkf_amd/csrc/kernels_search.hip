@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
               const int64_t pos = cur + lane_rank(kmask);
               if (pos < lim) {
                 C.list_idx[pos] = (int32_t)s;
-                C.list_w[pos] = w;
+                C.list_w[pos] = C.list_mode ? sqrtf(w / O.err[s]) : w;
               }
             }
             if (lane == 0) s_cnt[wid][g] = cur + __popcll(kmask);

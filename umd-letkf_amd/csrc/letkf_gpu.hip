@@ -98,6 +98,8 @@ struct letkf_gpu_t {
   int64_t* h_tot = nullptr;    // pinned
 
   letkf_gpu_timing timing{};
+  int tune_xcd = 0;       // XCD-contiguous block order in the product kernels
+  int tune_bucket = 1;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
 
   bool batched() const {
     return dims.path == LETKF_PATH_BATCHED && dims.precision != LETKF_PREC_FP64;
@@ -168,6 +170,8 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   h->k = d->ens_size;
   h->ns = d->grid_ns;
   h->nij = d->ij_count;
+  if (const char* e = getenv("LETKF_GPU_XCD")) h->tune_xcd = atoi(e);
+  if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = atoi(e);
   hipGetDevice(&h->device);
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
@@ -512,6 +516,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
   if (h->eigK) eigen_caps((int)k, &h->cap_rot, &h->cap_it);
   const int64_t per_pair = 8 * k * k + 8 * k + 3 * 8 * (int64_t)h->eigK +
                            8 * (int64_t)h->eigK * h->eigK +
+
                            16 * (int64_t)h->cap_rot + 16 * (int64_t)h->cap_it + 32;
   int64_t pairs = std::max<int64_t>(1, std::min<int64_t>(131072, (12ll << 30) / per_pair));
   int64_t cg = std::max<int64_t>(1, pairs / G);
@@ -595,7 +600,8 @@ int letkf_gpu_run(letkf_gpu_t* h) {
   for (int64_t i0 = 0; i0 < h->nij; i0 += h->chunk_gp) {
     const int64_t ngp = std::min(h->chunk_gp, h->nij - i0);
     const int64_t np = ngp * G;
-    ChunkDev C{i0, ngp, (int)G, h->d_cnt, h->d_off, h->d_solve, h->d_list_idx, h->d_list_w};
+    ChunkDev C{i0, ngp, (int)G, h->d_cnt, h->d_off, h->d_solve, h->d_list_idx, h->d_list_w,
+               h->batched() ? 1 : 0, h->tune_xcd};
     launch_search(st, false, h->loc, O, S, V, C, D);
     HIPCHK(h, hipGetLastError());
     size_t tb = h->tmp_bytes;
@@ -623,7 +629,7 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     HIPCHK(h, hipGetLastError());
     rec();
     if (h->batched()) {
-      launch_bucket_keys(st, np, h->d_cnt, h->d_solve, h->d_bkey, h->d_bval);
+      launch_bucket_keys(st, h->tune_bucket, np, h->d_cnt, h->d_solve, h->d_bkey, h->d_bval);
       size_t sb = h->stmp_bytes;
       HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(h->d_stmp, sb, h->d_bkey, h->d_bkey2,
                                                    h->d_bval, h->d_perm, (int)np, 0, 32, st));
@@ -752,7 +758,7 @@ int letkf_gpu_get_selection(letkf_gpu_t* h, int64_t ij, int32_t group, int32_t* 
     return fail(h, LETKF_EINVAL, "gridpoint / group out of range");
   // re-run the search of the chunk that holds ij
   const int64_t i0 = ij;
-  ChunkDev C{i0, 1, h->G, h->d_cnt, h->d_off, h->d_solve, h->d_list_idx, h->d_list_w};
+  ChunkDev C{i0, 1, h->G, h->d_cnt, h->d_off, h->d_solve, h->d_list_idx, h->d_list_w, 0, 0};
   StateDev S{h->k, h->ns, h->nij, h->d_state, h->d_mean, h->d_sprd, h->d_lat, h->d_lon,
              h->d_mask, h->d_gx, h->d_gy, h->d_gz};
   ObsDev O{h->nobs, h->d_ox, h->d_oy, h->d_oz, h->d_ozdim, h->d_oerr, h->d_odep,

@@ -78,8 +78,22 @@ struct ChunkDev {
   int64_t* off;        // list offset per pair
   uint8_t* solve;      // pair needs a transform
   int32_t* list_idx;   // sorted obs index
-  float* list_w;       // localization weight
+  float* list_w;       // localization weight, or sqrt(weight / err) (list_mode 1)
+  int list_mode;       // 0: rloc (FP64 products), 1: sqrt(rloc / err) (mixed)
+  int xcd;             // product kernels: XCD-contiguous block order
 };
+
+// XCD-aware block order: the dispatcher deals workgroups round-robin over the
+// 8 XCDs, each with its own L2; remap so that every XCD walks a contiguous
+// range of blocks (neighbouring gridpoints share observations).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+  constexpr int kXcd = 8;
+  const int64_t per = (nb + kXcd - 1) / kXcd;
+  const int64_t x = b % kXcd, i = b / kXcd;
+  const int64_t full = nb - (per - 1) * kXcd;     // XCDs that get `per` blocks
+  const int64_t base = x < full ? x * per : full * per + (x - full) * (per - 1);
+  return base + i;
+}
 
 struct DiagDev {
   float* maxhz;        // (nij)
@@ -133,8 +147,8 @@ void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
                             const double* bvec, const int* only_flagged, double* ana);
 void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev& S,
                        const ChunkDev& C, const int32_t* perm, double* B, double* bvec);
-void launch_bucket_keys(hipStream_t st, int64_t np, const int64_t* cnt, const uint8_t* solve,
-                        uint32_t* keys, int32_t* vals);
+void launch_bucket_keys(hipStream_t st, int mode, int64_t np, const int64_t* cnt,
+                        const uint8_t* solve, uint32_t* keys, int32_t* vals);
 int eigen_pipeline_K(int k);
 void eigen_caps(int k, int* cap_rot, int* cap_it);
 void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
