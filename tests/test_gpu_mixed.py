@@ -46,3 +46,22 @@ def test_random(prec, k):
 def test_localizers(prec, loc):
     _mixed(synth.random_problem(seed=300, k=40, ns=3, nobs=3000, loc_cls=loc,
                                 qc_like_times=(loc == "LOC_OCEAN")), prec)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("k,ns", [(20, 3), (50, 1), (31, 8)])
+def test_rational_and_eigen_paths_agree(prec, k, ns, monkeypatch):
+    """The batched path applies A^(-1/2) by the rational approximation (no
+    eigensolve, the default when every group has at most 8 slabs); with
+    LETKF_GPU_RAT=0 it runs the QL eigensolve instead.  Same products, so the
+    two agree far below the precision of the products, and both meet the
+    tolerance against the oracle."""
+    p = synth.random_problem(seed=400 + k + ns, k=k, ns=ns, nobs=3000, rtps=0.3)
+    out = {}
+    for rat in ("1", "0"):
+        monkeypatch.setenv("LETKF_GPU_RAT", rat)
+        out[rat] = run_gpu(p, precision=prec, path="batched")
+    d = {f: rel(out["1"][f], out["0"][f]) for f in ("mean", "sprd", "state")}
+    print(prec, k, ns, d, flush=True)
+    assert max(d.values()) <= 1e-9, d
+    _mixed(p, prec)

@@ -71,6 +71,21 @@ def test_jacobi_and_ql_solvers_agree(k, monkeypatch):
         print(mode, k, errs, flush=True)
 
 
+@pytest.mark.parametrize("vec", ["0", "1"])
+@pytest.mark.parametrize("case", ["k20_ns3", "k50_ns1", "k31_ns8", "vgauss"])
+def test_vector_and_full_eigen_paths(case, vec, monkeypatch):
+    """Both ways of applying the transform meet the 1e-10 bar: eigenvectors
+    formed and applied (LETKF_GPU_VEC=0), and the rotations applied to the
+    carried vectors only (the default when every group has at most 8 slabs)."""
+    monkeypatch.setenv("LETKF_GPU_VEC", vec)
+    if case == "vgauss":
+        p = synth.random_problem(seed=61, k=40, ns=4, nobs=2500, loc_cls="LOC_VGAUSS", rtps=0.4)
+    else:
+        k, ns = {"k20_ns3": (20, 3), "k50_ns1": (50, 1), "k31_ns8": (31, 8)}[case]
+        p = synth.random_problem(seed=60 + k + ns, k=k, ns=ns, nobs=2500, rtpp=0.3)
+    _check(p)
+
+
 def test_k_above_pipeline_uses_jacobi():
     p = synth.random_problem(seed=77, k=70, ns=1, nobs=1500)
     _check(p)

@@ -20,6 +20,7 @@
 
 #include "det_math.h"
 #include "letkf_internal.h"
+#include "ratnodes.h"
 
 using namespace lk;
 
@@ -87,6 +88,14 @@ struct letkf_gpu_t {
   int4* d_itr = nullptr;
   int *d_nit = nullptr, *d_status = nullptr;
   double* d_Z = nullptr;
+  // vector path: carried vectors (NV x K per problem) and slab counts
+  double* d_U = nullptr;
+  int* d_nvec = nullptr;
+  int vec_nv = 0, alloc_nv = -1;
+  // batched path: node tables of the rational A^(-1/2)
+  double* d_rat = nullptr;
+  int* d_ratn = nullptr;
+  bool use_rat = false;
   // batched path: longest-first order of the problems of a chunk
   uint32_t *d_bkey = nullptr, *d_bkey2 = nullptr;
   int32_t *d_bval = nullptr, *d_perm = nullptr;
@@ -100,6 +109,8 @@ struct letkf_gpu_t {
   letkf_gpu_timing timing{};
   int tune_xcd = 0;       // XCD-contiguous block order in the product kernels
   int tune_bucket = 1;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
+  int tune_vec = 1;       // vector path when every group has few slabs
+  int tune_rat = 1;       // batched vector path: rational A^(-1/2), no eigensolve
 
   bool batched() const {
     return dims.path == LETKF_PATH_BATCHED && dims.precision != LETKF_PREC_FP64;
@@ -172,6 +183,8 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   h->nij = d->ij_count;
   if (const char* e = getenv("LETKF_GPU_XCD")) h->tune_xcd = atoi(e);
   if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = atoi(e);
+  if (const char* e = getenv("LETKF_GPU_VEC")) h->tune_vec = atoi(e);
+  if (const char* e = getenv("LETKF_GPU_RAT")) h->tune_rat = atoi(e);
   hipGetDevice(&h->device);
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
@@ -203,7 +216,8 @@ int letkf_gpu_destroy(letkf_gpu_t* h) {
   dfree(h->d_cnt); dfree(h->d_off); dfree(h->d_solve); dfree(h->d_list_idx);
   dfree(h->d_list_w); dfree(h->d_B); dfree(h->d_b); dfree(h->d_tmp);
   dfree(h->d_tau); dfree(h->d_dd); dfree(h->d_ee); dfree(h->d_rot); dfree(h->d_itr);
-  dfree(h->d_nit); dfree(h->d_status); dfree(h->d_Z);
+  dfree(h->d_nit); dfree(h->d_status); dfree(h->d_Z); dfree(h->d_U); dfree(h->d_nvec);
+  dfree(h->d_rat); dfree(h->d_ratn);
   dfree(h->d_bkey); dfree(h->d_bkey2); dfree(h->d_bval); dfree(h->d_perm); dfree(h->d_stmp);
   if (h->h_tot) hipHostFree(h->h_tot);
   hipStreamDestroy(h->stream);
@@ -491,6 +505,30 @@ int letkf_gpu_maxgroups(const letkf_gpu_t* h, int32_t* g) {
   return LETKF_OK;
 }
 
+// largest number of slabs one (gridpoint, group) problem can hold
+// (slab_group: LOC_VGAUSS groups by level, LOC_OCEAN bkg_t splits the
+// variable slabs at the mixed layer, every other case has one group)
+static int max_slabs_per_group(const letkf_gpu_t* h) {
+  if (h->loc.cls == LETKF_LOC_VGAUSS) {
+    std::vector<int> per(h->G, 0);
+    int mx = 0;
+    for (int64_t s = 0; s < h->ns; ++s) {
+      int lev = 0, var = -1;
+      for (int v = 0; v < h->nvars; ++v)
+        if (s >= h->var_s[v] && s < h->var_s[v] + h->var_lev[v]) { var = v; lev = (int)(s - h->var_s[v]); }
+      (void)var;
+      if (lev < h->G) mx = std::max(mx, ++per[lev]);
+    }
+    return mx;
+  }
+  if (h->loc.cls == LETKF_LOC_OCEAN && h->loc.vtloc_surf == 1) {
+    int n = 0;
+    for (int v = 0; v < h->nvars; ++v) n += h->var_lev[v];
+    return n;
+  }
+  return (int)h->ns;
+}
+
 static int ensure_outputs(letkf_gpu_t* h) {
   const int64_t k = h->k, ns = h->ns, nij = h->nij, G = h->G;
   if (!h->d_ana) {
@@ -503,22 +541,26 @@ static int ensure_outputs(letkf_gpu_t* h) {
     HIPCHK(h, dalloc(&h->d_maxcount, 1));
     HIPCHK(h, dalloc(&h->d_stats, 2));
   }
-  if (h->alloc_G == G) return LETKF_OK;
-  HIPCHK(h, dalloc(&h->d_lgm, (size_t)(G * nij)));
-  HIPCHK(h, dalloc(&h->d_lgloc, (size_t)(G * nij)));
-  HIPCHK(h, dalloc(&h->d_lghash, (size_t)(G * nij)));
-  // chunk sizing: bound the per-pair work arrays to about 12 GB
   // LETKF_GPU_EIGEN=jacobi selects the parallel Jacobi solver for every
   // problem (reference solver for tests); default: tridiagonal QL pipeline
   const char* ev = getenv("LETKF_GPU_EIGEN");
   const bool force_jacobi = ev && strcmp(ev, "jacobi") == 0;
-  h->eigK = force_jacobi ? 0 : eigen_pipeline_K((int)k);
-  if (h->eigK) eigen_caps((int)k, &h->cap_rot, &h->cap_it);
+  const int eigK = force_jacobi ? 0 : eigen_pipeline_K((int)k);
+  const int nv = (eigK && h->tune_vec) ? eigen_vec_nv(max_slabs_per_group(h)) : 0;
+  if (h->alloc_G == G && h->eigK == eigK && h->alloc_nv == nv) return LETKF_OK;
+  h->eigK = eigK;
+  h->vec_nv = nv;
+  HIPCHK(h, dalloc(&h->d_lgm, (size_t)(G * nij)));
+  HIPCHK(h, dalloc(&h->d_lgloc, (size_t)(G * nij)));
+  HIPCHK(h, dalloc(&h->d_lghash, (size_t)(G * nij)));
+  // chunk sizing: bound the per-pair work arrays to about 12 GB
+  h->use_rat = nv > 0 && h->batched() && h->tune_rat && k >= 2;
+  h->cap_rot = h->cap_it = 0;
+  if (h->eigK && !h->use_rat) eigen_caps((int)k, &h->cap_rot, &h->cap_it);
   const int64_t per_pair = 8 * k * k + 8 * k + 3 * 8 * (int64_t)h->eigK +
-                           8 * (int64_t)h->eigK * h->eigK +
-
-                           16 * (int64_t)h->cap_rot + 16 * (int64_t)h->cap_it + 32;
-  int64_t pairs = std::max<int64_t>(1, std::min<int64_t>(131072, (12ll << 30) / per_pair));
+                           (nv ? 8 * (int64_t)nv * h->eigK : 8 * (int64_t)h->eigK * h->eigK) +
+                           16 * (int64_t)h->cap_rot + 16 * (int64_t)h->cap_it + 40;
+  int64_t pairs = std::max<int64_t>(1, std::min<int64_t>(524288, (12ll << 30) / per_pair));
   int64_t cg = std::max<int64_t>(1, pairs / G);
   cg = std::min<int64_t>(cg, std::max<int64_t>(nij, 1));
   h->chunk_gp = cg;
@@ -536,7 +578,14 @@ static int ensure_outputs(letkf_gpu_t* h) {
     HIPCHK(h, dalloc(&h->d_rot, (size_t)(np * h->cap_rot)));
     HIPCHK(h, dalloc(&h->d_itr, (size_t)(np * h->cap_it)));
     HIPCHK(h, dalloc(&h->d_nit, (size_t)np));
-    HIPCHK(h, dalloc(&h->d_Z, (size_t)(np * h->eigK * h->eigK)));
+    if (nv) {
+      dfree(h->d_Z);
+      HIPCHK(h, dalloc(&h->d_U, (size_t)(np * nv * h->eigK)));
+      HIPCHK(h, dalloc(&h->d_nvec, (size_t)np));
+    } else {
+      dfree(h->d_U);
+      HIPCHK(h, dalloc(&h->d_Z, (size_t)(np * h->eigK * h->eigK)));
+    }
   }
   size_t tb = 0;
   HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, h->d_cnt, h->d_off, (int)np, h->stream));
@@ -553,7 +602,24 @@ static int ensure_outputs(letkf_gpu_t* h) {
     HIPCHK(h, dalloc((char**)&h->d_stmp, sb));
     h->stmp_bytes = sb;
   }
+  if (h->use_rat && !h->d_rat) {
+    // x^(-1/2) on [1, 2^b]: the fewest nodes (<= 31) for 1e-12 relative error,
+    // else 31 nodes
+    std::vector<double> tab((size_t)kRatBuckets * 64, 0.0);
+    std::vector<int> tn(kRatBuckets, 0);
+    for (int b = 1; b < kRatBuckets; ++b) {
+      double* sb = tab.data() + (size_t)b * 64;
+      int n = rat::choose(std::ldexp(1.0, b), 1e-12, 31, sb, sb + 32);
+      if (n < 0) { n = 31; rat::nodes(std::ldexp(1.0, b), n, sb, sb + 32); }
+      tn[b] = n;
+    }
+    HIPCHK(h, dalloc(&h->d_rat, tab.size()));
+    HIPCHK(h, dalloc(&h->d_ratn, tn.size()));
+    HIPCHK(h, hipMemcpy(h->d_rat, tab.data(), 8 * tab.size(), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_ratn, tn.data(), 4 * tn.size(), hipMemcpyHostToDevice));
+  }
   h->alloc_G = G;
+  h->alloc_nv = nv;
   return LETKF_OK;
 }
 
@@ -640,9 +706,11 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     HIPCHK(h, hipGetLastError());
     rec();
     if (h->eigK) {
+      RatTab R;
+      if (h->use_rat) { R.tab = h->d_rat; R.n = h->d_ratn; R.nb = kRatBuckets; }
       launch_eigen_pipeline(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_tau, h->d_dd,
                             h->d_ee, h->d_rot, h->cap_rot, h->d_itr, h->cap_it, h->d_nit,
-                            h->d_status, h->d_Z, h->d_ana);
+                            h->d_status, h->d_Z, h->vec_nv, h->d_U, h->d_nvec, R, h->d_ana);
       HIPCHK(h, hipGetLastError());
       // problems the QL could not finish go to the Jacobi solver
       launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_status, h->d_ana);

@@ -151,11 +151,27 @@ void launch_bucket_keys(hipStream_t st, int mode, int64_t np, const int64_t* cnt
                         const uint8_t* solve, uint32_t* keys, int32_t* vals);
 int eigen_pipeline_K(int k);
 void eigen_caps(int k, int* cap_rot, int* cap_it);
+// vectors carried per problem on the vector path (b + slabs of a group,
+// rounded up to an instantiated count), 0 if maxslab is too large
+int eigen_vec_nv(int maxslab);
+// node tables of the rational approximation of x^(-1/2) on [1, 2^b]:
+// tab[b * 64 + j] = s_j, tab[b * 64 + 32 + j] = w_j (j < n[b] <= 31)
+struct RatTab {
+  const double* tab = nullptr;
+  const int* n = nullptr;
+  int nb = 0;                 // 0: not used
+};
+constexpr int kRatBuckets = 48;
+// NV == 0: eigenvectors formed (Z, V = QZ) and applied to every slab;
+// NV > 0: only the NV carried vectors are transformed (U: NV x K per
+// problem), by the QL rotations, or, with a node table (R.nb > 0), by the
+// rational approximation of A^(-1/2) without an eigensolve
 void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
                            const VarsDev& V, const ChunkDev& C, const DiagDev& D,
                            double* B, const double* bvec, double* tau, double* dd,
                            double* ee, double2* rot, int cap_rot, int4* itr,
-                           int cap_it, int* nit, int* status, double* Z, double* ana);
+                           int cap_it, int* nit, int* status, double* Z, int NV, double* U,
+                           int* nvec, const RatTab& R, double* ana);
 void launch_finalize(hipStream_t st, const LocDev& L, const StateDev& S,
                      const VarsDev& V, const ChunkDev& C, const DiagDev& D,
                      float rtps, float rtpp, float mul, const double* ana_in,
