@@ -1,8 +1,10 @@
-"""Build libletkf_gpu.so in-tree with hipcc for gfx950.
+"""Build libletkf_gpu.so in-tree with hipcc for gfx950, and the Fortran host.
 
 Every .hip source under ``umd-letkf_amd/csrc`` is compiled to an object in
 ``umd-letkf_amd/_build`` (in parallel) and linked into
-``umd-letkf_amd/letkf_amd/libletkf_gpu.so``.
+``umd-letkf_amd/letkf_amd/libletkf_gpu.so``.  ``build_fortran`` compiles the
+iso_c_binding module, the solver host module and the stand-alone driver in
+``umd-letkf_amd/fortran`` with amdflang into ``umd-letkf_amd/fortran/_build``.
 """
 from __future__ import annotations
 
@@ -61,5 +63,41 @@ def build(verbose: bool = True) -> str:
     return OUT
 
 
+FORTRAN = os.path.join(ROOT, "fortran")
+FBUILD = os.path.join(FORTRAN, "_build")
+DRIVER = os.path.join(FBUILD, "letkf_gpu_driver")
+
+
+def amdflang() -> str:
+    for c in (os.environ.get("FC"), "/opt/rocm/bin/amdflang", shutil.which("amdflang")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("amdflang not found")
+
+
+def build_fortran(verbose: bool = True) -> str:
+    """Compile letkf_gpu_mod, letkf_solver_gpu and letkf_gpu_driver; the
+    driver links libletkf_gpu.so through an $ORIGIN-relative rpath."""
+    os.makedirs(FBUILD, exist_ok=True)
+    objs = []
+    for src in ("letkf_gpu_mod.F90", "letkf_solver_gpu.F90", "letkf_gpu_driver.F90"):
+        obj = os.path.join(FBUILD, src.replace(".F90", ".o"))
+        cmd = [amdflang(), "-O2", "-c", os.path.join(FORTRAN, src), "-module-dir", FBUILD,
+               "-I", FBUILD, "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {src}:\n{r.stderr[-4000:]}")
+        objs.append(obj)
+    cmd = [amdflang(), *objs, "-L", PKG, "-lletkf_gpu",
+           "-Wl,-rpath,$ORIGIN/../../letkf_amd", "-o", DRIVER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran link failed:\n{r.stderr[-4000:]}")
+    if verbose:
+        print(f"built {DRIVER}")
+    return DRIVER
+
+
 if __name__ == "__main__":
     build()
+    build_fortran()
