@@ -6,10 +6,12 @@ Workload (default): the 0.25-degree 1440x720 surface analysis with k = 50,
 (seeded synthetic data, inputs resident in HBM before the timed region).
 A step is one full analysis (letkf_gpu_run) over all 1,036,800 gridpoints.
 
-Printed: one JSON line with the metric, the roofline of the dominant kernel
-(from the library's HIP events on its own stream), the CPU baseline (the
-FP64 oracle on a bounded sample of gridpoints, all host threads up to 16)
-and the error of the GPU analysis against that oracle on the same sample.
+Printed: one JSON line with the metric, the roofline of the product kernel
+(the largest single kernel; from the library's HIP events on its own
+stream, with the HBM traffic recorded from rocprofv3 counters in
+profiles/traffic.json), the CPU baseline (the FP64 oracle on a bounded
+sample of gridpoints, all host threads up to 16) and the error of the GPU
+analysis against that oracle on the same sample.
 """
 import argparse
 import json
@@ -43,27 +45,114 @@ def make_problem(name):
     raise SystemExit(f"unknown config {name}")
 
 
-def roofline(t, k, precision, nsteps):
-    """Algorithmic work of the product and eigen phases per step."""
-    prod_flop = (2.0 * k * k + 2.0 * k) * t["n_pairs"]
-    eig_flop = t["n_problems"] * (9.0 * k ** 3 + 2.0 * k ** 3 + 4.0 * k * k)
-    phases = {}
-    for name, flop, peak in (("product", prod_flop, PEAK[precision]),
-                             ("eigen", eig_flop, PEAK["fp64"])):
-        ms = t[f"{name}_ms"]
-        ach = flop / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
-        phases[name] = dict(ms=ms, launches=t["n_launch"],
-                            avg_launch_ms=ms / max(t["n_launch"], 1),
-                            flop=flop, achieved=ach, peak=peak, frac=ach / peak)
-    phases["search"] = dict(ms=t["search_ms"], candidates=t["n_cand"])
-    phases["finalize"] = dict(ms=t["finalize_ms"])
-    dom = max(("product", "eigen"), key=lambda n: phases[n]["ms"])
-    p = phases[dom]
-    roof = dict(kernel=dom, bound="mfma" if dom == "product" else "fp64-valu",
-                achieved=round(p["achieved"], 4), peak=p["peak"], unit="TFLOP/s",
-                frac=round(p["frac"], 5), traffic=None,
-                work=f"{p['flop']:.4e} flop per step over {p['launches']} launches")
-    return roof, phases
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC_FILE = os.path.join(PROFILES, "traffic.json")
+# rocprofv3 counter scale factors measured on a 1 GiB device copy (FETCH_SIZE
+# read 524308 KiB for 1048576 KiB moved; WRITE_SIZE read 1048576 KiB)
+FETCH_SCALE = 2.0
+WRITE_SCALE = 1.0
+
+
+def product_kernel(path, precision):
+    return "k_product_f64" if path == "plain" else "k_product_mx"
+
+
+def roofline(t, k, path, precision, workload):
+    """Roofline of the product kernel (the largest single kernel): the
+    algorithmic flops F_prod = sum over problems of 2k^2 m + 2k m per launch
+    over the kernel's own HIP-event time on the library stream."""
+    flop = (2.0 * k * k + 2.0 * k) * t["n_pairs"]
+    ms = t["product_ms"]
+    nl = max(int(t["n_launch"]), 1)
+    peak = PEAK["fp64"] if path == "plain" else PEAK[precision]
+    ach = flop / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+    kern = product_kernel(path, precision)
+    gather = t["n_pairs"] * (4.0 * k + 8.0)          # hx column + (index, weight)
+    roof = dict(kernel=kern, bound="mfma", achieved=round(ach, 3), peak=peak, unit="TFLOP/s",
+                frac=round(ach / peak, 5), traffic=None,
+                launches=nl, avg_launch_ms=round(ms / nl, 4),
+                algorithmic=f"{flop / nl:.4e} flop and {gather / nl:.4e} gathered bytes per launch",
+                gather_GBps=round(gather / (ms * 1e-3) / 1e9, 1) if ms > 0 else 0.0)
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        ent = tr.get(f"{workload}/{path}/{precision}")
+        if ent and ent.get("kernel") == kern:
+            roof["traffic"] = ent["bytes_per_dispatch"]
+            roof["traffic_note"] = (f"HBM bytes per dispatch from rocprofv3 --pmc FETCH_SIZE x "
+                                    f"{FETCH_SCALE} + WRITE_SIZE x {WRITE_SCALE} "
+                                    f"({ent['source']}); launches of {ent['problems_per_dispatch']} "
+                                    f"problems")
+    except (OSError, ValueError):
+        pass
+    return roof
+
+
+def summarize_pmc(fetch_csv, write_csv, workload, path, precision, problems):
+    """Per-dispatch HBM traffic of the product kernel from two rocprofv3
+    --pmc runs, merged into profiles/traffic.json."""
+    import csv
+    kern = product_kernel(path, precision)
+
+    def per_dispatch(fn, counter):
+        vals = [float(r["Counter_Value"]) for r in csv.DictReader(open(fn))
+                if r["Counter_Name"] == counter and kern in r["Kernel_Name"]]
+        if not vals:
+            raise SystemExit(f"no {counter} rows for {kern} in {fn}")
+        return sum(vals) / len(vals), len(vals)
+    fk, n1 = per_dispatch(fetch_csv, "FETCH_SIZE")
+    wk, n2 = per_dispatch(write_csv, "WRITE_SIZE")
+    ent = dict(kernel=kern, fetch_kib=fk, write_kib=wk, dispatches=[n1, n2],
+               bytes_per_dispatch=round(1024.0 * (fk * FETCH_SCALE + wk * WRITE_SCALE)),
+               problems_per_dispatch=problems,
+               source=f"{os.path.basename(os.path.dirname(fetch_csv))}, "
+                      f"{os.path.basename(os.path.dirname(write_csv))}")
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+    except (OSError, ValueError):
+        tr = {}
+    tr[f"{workload}/{path}/{precision}"] = ent
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(tr, f, indent=1, sort_keys=True)
+    print(json.dumps(ent))
+
+
+def cpu_baseline(prob, solver, seconds, nthreads):
+    """The FP64 oracle on a bounded random sample of unmasked gridpoints.
+    The oracle's search-structure set-up is timed apart and left out, like
+    the GPU's set_obs is left out of the GPU's timed region."""
+    import oracle as O
+    g = np.random.default_rng(1234)
+    pool = np.nonzero(prob.mask == 0)[0]
+    pilot = np.sort(g.choice(pool, min(256, pool.size), replace=False))
+    r = O.analysis(prob, points=pilot, nthreads=nthreads)
+    per = r["loop_s"] / pilot.size
+    n = int(min(pool.size, max(256, seconds / max(per, 1e-7))))
+    pts = np.sort(g.choice(pool, n, replace=False))
+    orc = O.analysis(prob, points=pts, nthreads=nthreads)
+    base = {"value": round(n / orc["loop_s"], 2), "unit": "gridpoints/s", "cores": nthreads,
+            "kind": "port",
+            "sample": f"{n} random unmasked gridpoints of the same analysis, FP64 oracle, "
+                      f"gridpoint loop {orc['loop_s']:.1f} s (+ {orc['setup_s']:.1f} s search "
+                      f"set-up, not counted)"}
+    st, mn, sp = solver.state_f64()
+    d = solver.diag()
+    sel_ok = bool(np.array_equal(d["lg_selhash"][pts, :].T, orc["lg_hash"]) and
+                  np.array_equal(d["lg_obscount"][pts, :].T.astype(np.int64), orc["lg_m"]))
+
+    def rel(a, b):
+        den = np.abs(b).max()
+        return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
+    inc_o = orc["mean"] - prob.mean[:, pts]
+    pert_o = orc["state"] - orc["mean"][None]
+    err = {"sample_gridpoints": n, "selection_bit_exact": sel_ok,
+           "mean_rel": rel(mn[:, pts], orc["mean"]), "sprd_rel": rel(sp[:, pts], orc["sprd"]),
+           "members_rel": rel(st[:, :, pts], orc["state"]),
+           "perturbations_rel": rel(st[:, :, pts] - mn[None, :, pts], pert_o),
+           "increment_rel": rel(mn[:, pts] - prob.mean[:, pts], inc_o),
+           "mean_maxabs": float(np.abs(mn[:, pts] - orc["mean"]).max())}
+    return base, err
 
 
 def main():
@@ -78,7 +167,15 @@ def main():
                     help="skip timing the plain FP64 path on the same workload")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="record the product kernel's per-dispatch HBM traffic from two "
+                         "rocprofv3 --pmc counter_collection.csv files in profiles/traffic.json")
+    ap.add_argument("--problems-per-dispatch", type=int, default=0)
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, args.config, args.path, args.precision,
+                      args.problems_per_dispatch)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; "
                           "the analysis shards by gridpoint across GPUs (see DESIGN.md)"}))
@@ -90,8 +187,10 @@ def main():
     prob, workload = make_problem(args.config)
     log(f"[bench] problem {args.config}: nij={prob.nij} k={prob.k} nobs={prob.nobs} "
         f"({time.time() - t0:.1f} s)")
+    tu = time.perf_counter()
     solver = LetkfGpuSolver(prob, precision=args.precision, path=args.path)
-    log("[bench] inputs uploaded")
+    upload_s = time.perf_counter() - tu
+    log(f"[bench] inputs uploaded ({upload_s:.2f} s)")
     for i in range(args.warmup):
         solver.run()
         log(f"[bench] warmup {i} done")
@@ -110,7 +209,7 @@ def main():
     tavg = {kk: v / args.steps for kk, v in ph_acc.items()}
     ms = 1e3 * elapsed / args.steps
     value = prob.nij * args.steps / elapsed
-    roof, phases = roofline(tavg, prob.k, args.precision, args.steps)
+    roof = roofline(tavg, prob.k, args.path, args.precision, args.config)
 
     out = {
         "metric": "analysis gridpoints/sec at k=50, batched mixed-prec vs FP64 GPU; rel err vs FP64",
@@ -123,45 +222,21 @@ def main():
                    "path": args.path, "precision": args.precision,
                    "parallelism": "single GPU"},
         "roofline": roof,
-        "phases_ms": {kk: round(v["ms"], 3) for kk, v in phases.items()},
+        "phases_ms": {"search": round(tavg["search_ms"], 3),
+                      "order": round(tavg["order_ms"], 3),
+                      "product": round(tavg["product_ms"], 3),
+                      "transform": round(tavg["eigen_ms"], 3),
+                      "finalize": round(tavg["finalize_ms"], 3)},
         "work": {"solves": int(tavg["n_problems"]), "kept_pairs": int(tavg["n_pairs"]),
-                 "candidates": int(tavg["n_cand"])},
+                 "candidates": int(tavg["n_cand"]), "chunks": int(tavg["n_launch"])},
+        "upload_s": round(upload_s, 3),
+        "pcie_inclusive_gridpoints_per_s": round(prob.nij / (upload_s + ms * 1e-3), 1),
     }
 
     if not args.no_cpu:
-        import oracle as O
         nt = max(1, min(16, os.cpu_count() or 1))
-        g = np.random.default_rng(1234)
-        pool = np.nonzero(prob.mask == 0)[0]
-        pilot = np.sort(g.choice(pool, min(64, pool.size), replace=False))
-        tp = time.perf_counter()
-        O.analysis(prob, points=pilot, nthreads=nt)
-        per = (time.perf_counter() - tp) / pilot.size
-        n = int(min(pool.size, max(64, args.cpu_seconds / max(per, 1e-6))))
-        pts = np.sort(g.choice(pool, n, replace=False))
-        tc = time.perf_counter()
-        orc = O.analysis(prob, points=pts, nthreads=nt)
-        dt = time.perf_counter() - tc
-        out["cpu_baseline"] = {"value": round(n / dt, 2), "unit": "gridpoints/s", "cores": nt,
-                               "kind": "port",
-                               "sample": f"{n} random unmasked gridpoints of the same "
-                                         f"analysis, FP64 oracle, {dt:.1f} s"}
-        st, mn, sp = solver.state_f64()
-        d = solver.diag()
-        sel_ok = bool(np.array_equal(d["lg_selhash"][pts, :].T, orc["lg_hash"]) and
-                      np.array_equal(d["lg_obscount"][pts, :].T.astype(np.int64), orc["lg_m"]))
-
-        def rel(a, b):
-            den = np.abs(b).max()
-            return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
-        inc_o = orc["mean"] - prob.mean[:, pts]
-        out["error_vs_fp64_oracle"] = {
-            "sample_gridpoints": n, "selection_bit_exact": sel_ok,
-            "mean_rel": rel(mn[:, pts], orc["mean"]), "sprd_rel": rel(sp[:, pts], orc["sprd"]),
-            "members_rel": rel(st[:, :, pts], orc["state"]),
-            "increment_rel": rel(mn[:, pts] - prob.mean[:, pts], inc_o),
-            "mean_maxabs": float(np.abs(mn[:, pts] - orc["mean"]).max()),
-        }
+        out["cpu_baseline"], out["error_vs_fp64_oracle"] = cpu_baseline(
+            prob, solver, args.cpu_seconds, nt)
     solver.final()
     if not args.no_fp64_compare and not (args.path == "plain" and args.precision == "fp64"):
         ref = LetkfGpuSolver(prob, precision="fp64", path="plain")
@@ -175,8 +250,10 @@ def main():
         ref.final()
         out["fp64_plain"] = {"value": round(prob.nij * args.steps / dt, 1),
                              "ms_per_step": round(1e3 * dt / args.steps, 3),
-                             "phases_ms": {kk: round(rt[f"{kk}_ms"], 3) for kk in
-                                           ("search", "product", "eigen", "finalize")}}
+                             "phases_ms": {"search": round(rt["search_ms"], 3),
+                                           "product": round(rt["product_ms"], 3),
+                                           "transform": round(rt["eigen_ms"], 3),
+                                           "finalize": round(rt["finalize_ms"], 3)}}
         out["speedup_vs_fp64_plain"] = round(out["value"] / out["fp64_plain"]["value"], 3)
         log(f"[bench] fp64 plain: {out['fp64_plain']}")
     print(json.dumps(out), flush=True)

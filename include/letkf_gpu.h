@@ -130,6 +130,8 @@ typedef struct letkf_gpu_timing {
   int64_t n_pairs;     /* kept (gridpoint, group, obs) triples        */
   int64_t n_cand;      /* candidate (gridpoint, obs) pairs tested     */
   int64_t n_launch;    /* launches per phase (gridpoint chunks)       */
+  double order_ms;     /* batched path: problem order (keys + sort); the
+                          product_ms above is then the product kernel alone */
 } letkf_gpu_timing;
 
 typedef struct letkf_gpu_t letkf_gpu_t;

@@ -28,6 +28,7 @@
 
 #include <math.h>
 #include <pthread.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -766,10 +767,18 @@ static void* worker(void* argp) {
   return NULL;
 }
 
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 int orc_analysis(const orc_problem* p, const int64_t* pts, int64_t npts,
                  int nthreads, const orc_output* out) {
   orc_ctx c;
+  const double t0 = now_s();
   if (ctx_init(&c, p)) { ctx_free(&c); return -1; }
+  const double t1 = now_s();
   if (nthreads < 1) nthreads = 1;
   if (nthreads > 256) nthreads = 256;
   pthread_t th[256];
@@ -783,6 +792,10 @@ int orc_analysis(const orc_problem* p, const int64_t* pts, int64_t npts,
   for (int t = 0; t < nthreads; ++t) {
     if (nthreads > 1) pthread_join(th[t], NULL);
     if (args[t].status) status = -1;
+  }
+  if (out->seconds) {
+    out->seconds[0] = t1 - t0;
+    out->seconds[1] = now_s() - t1;
   }
   ctx_free(&c);
   return status;

@@ -56,6 +56,7 @@ typedef struct orc_output {
   int64_t* lg_m;        /* (maxgroups, npts) */
   double* lg_obsloc;    /* (maxgroups, npts) */
   uint64_t* lg_hash;    /* (maxgroups, npts) */
+  double* seconds;      /* optional [2]: search-structure set-up, gridpoint loop */
 } orc_output;
 
 /* deterministic elementary functions (shared definition with the GPU) */

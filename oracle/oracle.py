@@ -38,7 +38,7 @@ class OrcProblem(C.Structure):
 class OrcOutput(C.Structure):
     _fields_ = [("state", _DP), ("mean", _DP), ("sprd", _DP), ("maxhz", _FP),
                 ("obscount", _LP), ("lg_m", _LP), ("lg_obsloc", _DP),
-                ("lg_hash", _UP)]
+                ("lg_hash", _UP), ("seconds", _DP)]
 
 
 def build():
@@ -219,11 +219,14 @@ def analysis(prob, points=None, nthreads=None):
     o.lg_m = _p(out["lg_m"], C.c_int64)
     o.lg_obsloc = _p(out["lg_obsloc"], C.c_double)
     o.lg_hash = _p(out["lg_hash"], C.c_uint64)
+    secs = np.zeros(2)
+    o.seconds = _p(secs, C.c_double)
     nt = nthreads or min(os.cpu_count() or 1, 16)
     rc = lib().orc_analysis(C.byref(b.s), _p(pts, C.c_int64), n, nt, C.byref(o))
     if rc != 0:
         raise RuntimeError("oracle analysis failed")
     out["points"] = pts
+    out["setup_s"], out["loop_s"] = float(secs[0]), float(secs[1])
     return out
 
 

@@ -3,6 +3,7 @@ declared in include/letkf_gpu.h, and the ctypes mirror matches the header.
 No compute call is made (no GPU here)."""
 import ctypes as C
 import os
+import subprocess
 import re
 
 from letkf_amd import abi
@@ -26,12 +27,24 @@ def test_exports_every_declared_symbol():
     assert lib.letkf_gpu_version().decode().startswith("letkf_gpu")
 
 
-def test_struct_layouts():
-    # sizes follow the header (natural C alignment)
-    assert C.sizeof(abi.Dims) == 40
-    assert C.sizeof(abi.LinearInterpLat) == 4 + 8 * abi.MAX_INTERP
-    assert C.sizeof(abi.Diag) == 5 * 8
-    assert C.sizeof(abi.Timing) == 5 * 8 + 4 * 8
+def test_struct_layouts(tmp_path):
+    """The ctypes mirrors have the C compiler's sizes and field offsets for
+    every struct of include/letkf_gpu.h."""
+    exe = str(tmp_path / "abi_sizes")
+    subprocess.run(["gcc", "-O0", os.path.join(ROOT, "tests", "native", "abi_sizes.c"),
+                    "-o", exe], check=True)
+    out = dict(ln.rsplit(" ", 1) for ln in
+               subprocess.run([exe], capture_output=True, text=True, check=True)
+               .stdout.splitlines())
+    mirror = {"letkf_gpu_dims": abi.Dims, "letkf_linearinterp_lat": abi.LinearInterpLat,
+              "letkf_loc_desc": abi.LocDesc, "letkf_gpu_diag": abi.Diag,
+              "letkf_gpu_timing": abi.Timing}
+    for name, cls in mirror.items():
+        assert C.sizeof(cls) == int(out[name]), name
+    for key, val in out.items():
+        if "." in key:
+            struct, field = key.split(".")
+            assert getattr(mirror[struct], field).offset == int(val), key
 
 
 def test_no_cpu_fallback_in_package():

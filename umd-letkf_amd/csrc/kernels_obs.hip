@@ -85,6 +85,25 @@ __global__ void k_obs_gather_hx(int64_t n, int k, const int32_t* __restrict__ pe
   }
 }
 
+// operand columns of the mixed-precision products: hxp(kp, n) in search
+// order, rows 0..k-1 the hx perturbations, row k the departure, rows above
+// zero, so a column can be read without row tests
+__global__ void k_obs_pad_hx(int64_t n, int k, int kp, const float* __restrict__ ohx,
+                             const float* __restrict__ odep, float* hxp) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * kp) return;
+  const int64_t s = e / kp;
+  const int a = (int)(e - s * kp);
+  hxp[e] = a < k ? ohx[s * k + a] : (a == k ? odep[s] : 0.0f);
+}
+
+void launch_obs_pad_hx(hipStream_t st, int64_t n, int k, int kp, const float* ohx,
+                       const float* odep, float* hxp) {
+  if (n <= 0) return;
+  const int64_t tot = n * kp;
+  k_obs_pad_hx<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(n, k, kp, ohx, odep, hxp);
+}
+
 // cell_start[c] = first sorted obs with cell >= c (c = 0..kCells)
 __global__ void k_cell_start(int64_t n, const int32_t* __restrict__ scell,
                              int32_t* cell_start) {

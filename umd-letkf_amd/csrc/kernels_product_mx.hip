@@ -39,44 +39,120 @@ __device__ __forceinline__ int obs_slot(int half, int e) {
   return PREC == LETKF_PREC_FP32 ? 2 * e + half : 8 * half + e;
 }
 
+// Operand registers of one 16-observation step, without row or range tests:
+// the kept list of a problem starts at a multiple of 8 entries and is
+// followed by at least 8 readable entries, and the padded columns hxp carry
+// the departure in row k and zeros above it.  Entries past m read column 0
+// with weight 0.
 template <int NB, int PREC>
-__device__ __forceinline__ void load_frag(MxFrag<NB>& F, const ObsDev& O, const ChunkDev& C,
-                                          int64_t off, int64_t m, int64_t j0, int k, int lane) {
+__device__ __forceinline__ void load_frag(MxFrag<NB>& F, const ObsDev& O,
+                                          __amdgpu_buffer_rsrc_t hxr, const ChunkDev& C,
+                                          int64_t off, int64_t m, int64_t j0, int lane) {
   const int half = lane >> 5, i = lane & 31;
+  int32_t sidx[8];
+  float w[8];
+  if (PREC == LETKF_PREC_FP32) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int64_t j = off + j0 + obs_slot<PREC>(half, e);
+      sidx[e] = C.list_idx[j];
+      w[e] = C.list_w[j];                 // sqrt(rloc / err), written by the search
+    }
+  } else {
+    const int4* pi = reinterpret_cast<const int4*>(C.list_idx + off + j0 + 8 * half);
+    const float4* pw = reinterpret_cast<const float4*>(C.list_w + off + j0 + 8 * half);
+    const int4 a0 = pi[0], a1 = pi[1];
+    const float4 b0 = pw[0], b1 = pw[1];
+    sidx[0] = a0.x; sidx[1] = a0.y; sidx[2] = a0.z; sidx[3] = a0.w;
+    sidx[4] = a1.x; sidx[5] = a1.y; sidx[6] = a1.z; sidx[7] = a1.w;
+    w[0] = b0.x; w[1] = b0.y; w[2] = b0.z; w[3] = b0.w;
+    w[4] = b1.x; w[5] = b1.y; w[6] = b1.z; w[7] = b1.w;
+  }
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    const int64_t j = j0 + obs_slot<PREC>(half, e);
-    int32_t s = 0;
-    float w = 0.0f;
-    if (j < m) {
-      s = C.list_idx[off + j];
-      w = C.list_w[off + j];            // sqrt(rloc / err), written by the search
-    }
+    const bool valid = j0 + obs_slot<PREC>(half, e) < m;
+    const float ww = valid ? w[e] : 0.0f;
+    // byte offset of the lane's row in the column; past the buffer's range
+    // (returns 0) for entries beyond m
+    const uint32_t vo = valid ? ((uint32_t)sidx[e] * (uint32_t)O.kp + (uint32_t)i) * 4u
+                              : 0x80000000u;
 #pragma unroll
-    for (int rb = 0; rb < NB; ++rb) {
-      const int a = 32 * rb + i;
-      float v = 0.0f;
-      if (j < m) {
-        if (a < k) v = O.hx[(int64_t)s * k + a];
-        else if (a == k) v = O.dep[s];
-      }
-      F.y[rb][e] = v * w;
-    }
+    for (int rb = 0; rb < NB; ++rb)
+      F.y[rb][e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(hxr, vo, 128 * rb, 0)) * ww;
   }
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// two values per conversion (v_cvt_pk_bf16_f32, round to nearest even)
 __device__ __forceinline__ bf16x8 to_bf16(const float* v) {
   bf16x8 r;
 #pragma unroll
-  for (int e = 0; e < 8; ++e) r[e] = (__bf16)v[e];
+  for (int p = 0; p < 4; ++p) {
+    const bf16x2 h = __builtin_convertvector((f32x2){v[2 * p], v[2 * p + 1]}, bf16x2);
+    r[2 * p] = h[0];
+    r[2 * p + 1] = h[1];
+  }
   return r;
 }
+
+// x = hi + lo + O(2^-16 |x|): hi = bf16(x), lo = bf16(x - hi); x - hi is exact
+// in FP32, and hi widens by a shift of the packed bits
 __device__ __forceinline__ void split_bf16(const float* v, bf16x8& hi, bf16x8& lo) {
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const __bf16 h = (__bf16)v[e];
-    hi[e] = h;
-    lo[e] = (__bf16)(v[e] - (float)h);
+  for (int p = 0; p < 4; ++p) {
+    const float x0 = v[2 * p], x1 = v[2 * p + 1];
+    const bf16x2 h = __builtin_convertvector((f32x2){x0, x1}, bf16x2);
+    const uint32_t hu = __builtin_bit_cast(uint32_t, h);
+    const float r0 = x0 - __uint_as_float(hu << 16);
+    const float r1 = x1 - __uint_as_float(hu & 0xffff0000u);
+    const bf16x2 l = __builtin_convertvector((f32x2){r0, r1}, bf16x2);
+    hi[2 * p] = h[0];
+    hi[2 * p + 1] = h[1];
+    lo[2 * p] = l[0];
+    lo[2 * p + 1] = l[1];
+  }
+}
+
+// acc += the products of one 16-observation step
+template <int NB, int PREC>
+__device__ __forceinline__ void mx_step(f32x16* acc, const MxFrag<NB>& cur) {
+  // both operands are sqrt(R^-1 rloc) y: A B^T = Yb^T R^-1 Yb, and row k
+  // (sqrt(R^-1 rloc) d) gives Yb^T R^-1 d in column k
+  if (PREC == LETKF_PREC_FP32) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      int b = 0;
+#pragma unroll
+      for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+        for (int cb = rb; cb < NB; ++cb, ++b)
+          acc[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(cur.y[rb][e], cur.y[cb][e], acc[b], 0, 0, 0);
+    }
+  } else if (PREC == LETKF_PREC_BF16) {
+    bf16x8 v16[NB];
+#pragma unroll
+    for (int rb = 0; rb < NB; ++rb) v16[rb] = to_bf16(cur.y[rb]);
+    int b = 0;
+#pragma unroll
+    for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+      for (int cb = rb; cb < NB; ++cb, ++b)
+        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(v16[rb], v16[cb], acc[b], 0, 0, 0);
+  } else {
+    bf16x8 hi[NB], lo[NB];
+#pragma unroll
+    for (int rb = 0; rb < NB; ++rb) split_bf16(cur.y[rb], hi[rb], lo[rb]);
+    int b = 0;
+#pragma unroll
+    for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+      for (int cb = rb; cb < NB; ++cb, ++b) {
+        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo[rb], hi[cb], acc[b], 0, 0, 0);
+        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[rb], lo[cb], acc[b], 0, 0, 0);
+        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[rb], hi[cb], acc[b], 0, 0, 0);
+      }
   }
 }
 
@@ -92,52 +168,24 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
   const int64_t m = C.cnt[pair];
   const int64_t off = C.off[pair];
   const int lane = threadIdx.x;
+  // padded operand columns; the host guarantees n * kp * 4 < 2^31
+  const __amdgpu_buffer_rsrc_t hxr = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)O.hxp, (short)0, (int)(O.n * O.kp * 4), 0x00020000);
 
   f32x16 acc[NBLK];
 #pragma unroll
   for (int b = 0; b < NBLK; ++b) acc[b] = (f32x16){};
 
-  MxFrag<NB> cur, nxt;
-  load_frag<NB, PREC>(cur, O, C, off, m, 0, k, lane);
-  for (int64_t j0 = 0; j0 < m; j0 += 16) {
-    if (j0 + 16 < m) load_frag<NB, PREC>(nxt, O, C, off, m, j0 + 16, k, lane);
-    // both operands are sqrt(R^-1 rloc) y: A B^T = Yb^T R^-1 Yb, and row k
-    // (sqrt(R^-1 rloc) d) gives Yb^T R^-1 d in column k
-    if (PREC == LETKF_PREC_FP32) {
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        int b = 0;
-#pragma unroll
-        for (int rb = 0; rb < NB; ++rb)
-#pragma unroll
-          for (int cb = rb; cb < NB; ++cb, ++b)
-            acc[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(cur.y[rb][e], cur.y[cb][e], acc[b], 0, 0, 0);
-      }
-    } else if (PREC == LETKF_PREC_BF16) {
-      bf16x8 v16[NB];
-#pragma unroll
-      for (int rb = 0; rb < NB; ++rb) v16[rb] = to_bf16(cur.y[rb]);
-      int b = 0;
-#pragma unroll
-      for (int rb = 0; rb < NB; ++rb)
-#pragma unroll
-        for (int cb = rb; cb < NB; ++cb, ++b)
-          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(v16[rb], v16[cb], acc[b], 0, 0, 0);
-    } else {
-      bf16x8 hi[NB], lo[NB];
-#pragma unroll
-      for (int rb = 0; rb < NB; ++rb) split_bf16(cur.y[rb], hi[rb], lo[rb]);
-      int b = 0;
-#pragma unroll
-      for (int rb = 0; rb < NB; ++rb)
-#pragma unroll
-        for (int cb = rb; cb < NB; ++cb, ++b) {
-          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo[rb], hi[cb], acc[b], 0, 0, 0);
-          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[rb], lo[cb], acc[b], 0, 0, 0);
-          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[rb], hi[cb], acc[b], 0, 0, 0);
-        }
-    }
-    cur = nxt;
+  // two fragment buffers in turn: the next step's loads are in flight while
+  // the current step's MFMAs run
+  MxFrag<NB> fa, fb;
+  load_frag<NB, PREC>(fa, O, hxr, C, off, m, 0, lane);
+  for (int64_t j0 = 0; j0 < m; j0 += 32) {
+    const bool second = j0 + 16 < m;
+    if (second) load_frag<NB, PREC>(fb, O, hxr, C, off, m, j0 + 16, lane);
+    mx_step<NB, PREC>(acc, fa);
+    if (j0 + 32 < m) load_frag<NB, PREC>(fa, O, hxr, C, off, m, j0 + 32, lane);
+    if (second) mx_step<NB, PREC>(acc, fb);
   }
 
   // 32x32 f32 accumulator: register r of lane l holds row 8(r/4) + 4(l/32) + r%4

@@ -231,10 +231,11 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
   }
 
   if (!FILL && lane == 0) {
-    unsigned long long nsolve = 0;
+    unsigned long long nsolve = 0, npair = 0;
     for (int g = 0; g < G; ++g) {
       const int64_t m = s_cnt[wid][g];
       nsolve += (m > 0 && group_has_slabs(L, g, mld)) ? 1 : 0;
+      npair += (m > 0 && group_has_slabs(L, g, mld)) ? (unsigned long long)m : 0ull;
       C.cnt[t * G + g] = m;
       C.solve[t * G + g] = (m > 0 && group_has_slabs(L, g, mld)) ? 1 : 0;
       D.lg_m[ij * G + g] = (int32_t)m;
@@ -244,9 +245,11 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     D.obscount[ij] = (int32_t)nin;
     D.maxhz[ij] = maxhz;
     D.mld[ij] = mld;
-    atomicMax(D.max_count, (int32_t)(nin > 0x7fffffff ? 0x7fffffff : nin));
-    atomicAdd(&D.stats[0], nsolve);
-    atomicAdd(&D.stats[1], (unsigned long long)ncand);
+    unsigned long long* c = D.ctr + (blockIdx.x & (kCtrSlots - 1)) * kCtrStride;
+    atomicMax(c + kCtrMax, (unsigned long long)nin);
+    atomicAdd(c + kCtrSolves, nsolve);
+    atomicAdd(c + kCtrCand, (unsigned long long)ncand);
+    atomicAdd(c + kCtrPairs, npair);
   }
 }
 

@@ -49,6 +49,8 @@ struct letkf_gpu_t {
   float *d_ozdim = nullptr, *d_oerr = nullptr, *d_odep = nullptr, *d_otime = nullptr;
   int32_t *d_oidx = nullptr, *d_oobsid = nullptr, *d_oplatid = nullptr;
   float* d_ohx = nullptr;
+  float* d_ohxp = nullptr;   // padded operand columns (kp, n)
+  int kp = 0;
   int32_t* d_cell_start = nullptr;
   uint8_t* d_osurf = nullptr;
   float *d_otwp = nullptr, *d_otws = nullptr;
@@ -67,8 +69,9 @@ struct letkf_gpu_t {
   // outputs
   double *d_ana = nullptr, *d_amean = nullptr, *d_asprd = nullptr;
   float* d_maxhz = nullptr;
-  int32_t *d_obscount = nullptr, *d_lgm = nullptr, *d_mld = nullptr, *d_maxcount = nullptr;
-  unsigned long long* d_stats = nullptr;
+  int32_t *d_obscount = nullptr, *d_lgm = nullptr, *d_mld = nullptr;
+  unsigned long long* d_ctr = nullptr;   // search counters, kCtrSlots x kCtrStride
+  unsigned long long* h_ctr = nullptr;   // pinned copy
   double* d_lgloc = nullptr;
   uint64_t* d_lghash = nullptr;
 
@@ -108,7 +111,7 @@ struct letkf_gpu_t {
 
   letkf_gpu_timing timing{};
   int tune_xcd = 0;       // XCD-contiguous block order in the product kernels
-  int tune_bucket = 1;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
+  int tune_bucket = 2;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
   int tune_vec = 1;       // vector path when every group has few slabs
   int tune_rat = 1;       // batched vector path: rational A^(-1/2), no eigensolve
 
@@ -190,7 +193,9 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
     delete h;
     return fail(nullptr, LETKF_EHIP, "hipStreamCreate failed");
   }
-  if (hipHostMalloc((void**)&h->h_tot, 4 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void**)&h->h_tot, 4 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&h->h_ctr, sizeof(unsigned long long) * kCtrSlots * kCtrStride,
+                    hipHostMallocDefault) != hipSuccess) {
     hipStreamDestroy(h->stream);
     delete h;
     return fail(nullptr, LETKF_ENOMEM, "pinned allocation failed");
@@ -206,12 +211,12 @@ int letkf_gpu_destroy(letkf_gpu_t* h) {
   dfree(h->d_mask); dfree(h->d_gx); dfree(h->d_gy); dfree(h->d_gz);
   dfree(h->d_ox); dfree(h->d_oy); dfree(h->d_oz); dfree(h->d_ozdim); dfree(h->d_oerr);
   dfree(h->d_odep); dfree(h->d_otime); dfree(h->d_oidx); dfree(h->d_oobsid);
-  dfree(h->d_oplatid); dfree(h->d_ohx); dfree(h->d_cell_start); dfree(h->d_osurf);
+  dfree(h->d_oplatid); dfree(h->d_ohx); dfree(h->d_ohxp); dfree(h->d_cell_start); dfree(h->d_osurf);
   dfree(h->d_otwp); dfree(h->d_otws);
   dfree(h->d_slab_var); dfree(h->d_slab_lev); dfree(h->d_inc_max); dfree(h->d_bounds);
   dfree(h->d_lev_depth);
   dfree(h->d_ana); dfree(h->d_amean); dfree(h->d_asprd); dfree(h->d_maxhz);
-  dfree(h->d_obscount); dfree(h->d_lgm); dfree(h->d_mld); dfree(h->d_maxcount); dfree(h->d_stats);
+  dfree(h->d_obscount); dfree(h->d_lgm); dfree(h->d_mld); dfree(h->d_ctr);
   dfree(h->d_lgloc); dfree(h->d_lghash);
   dfree(h->d_cnt); dfree(h->d_off); dfree(h->d_solve); dfree(h->d_list_idx);
   dfree(h->d_list_w); dfree(h->d_B); dfree(h->d_b); dfree(h->d_tmp);
@@ -220,6 +225,7 @@ int letkf_gpu_destroy(letkf_gpu_t* h) {
   dfree(h->d_rat); dfree(h->d_ratn);
   dfree(h->d_bkey); dfree(h->d_bkey2); dfree(h->d_bval); dfree(h->d_perm); dfree(h->d_stmp);
   if (h->h_tot) hipHostFree(h->h_tot);
+  if (h->h_ctr) hipHostFree(h->h_ctr);
   hipStreamDestroy(h->stream);
   delete h;
   return LETKF_OK;
@@ -328,6 +334,10 @@ int letkf_gpu_set_obs(letkf_gpu_t* h, int64_t n, const float* lat, const float* 
                     r_obsid, r_platid, r_hx, r_hxm, scell, h->d_ox, h->d_oy, h->d_oz,
                     h->d_ozdim, h->d_oerr, h->d_odep, h->d_oidx, h->d_oobsid,
                     h->d_oplatid, h->d_otime, h->d_ohx, h->d_cell_start);
+  OBSCHK(hipGetLastError());
+  h->kp = (int)(((k + 1) + 31) / 32 * 32);
+  OBSCHK(dalloc(&h->d_ohxp, nn * h->kp));
+  launch_obs_pad_hx(st, n, (int)k, h->kp, h->d_ohx, h->d_odep, h->d_ohxp);
   OBSCHK(hipGetLastError());
   OBSCHK(hipStreamSynchronize(st));
   cleanup();
@@ -529,6 +539,11 @@ static int max_slabs_per_group(const letkf_gpu_t* h) {
   return (int)h->ns;
 }
 
+static hipcub::TransformInputIterator<int64_t, PadCount, const int64_t*> padded_counts(
+    const int64_t* cnt) {
+  return hipcub::TransformInputIterator<int64_t, PadCount, const int64_t*>(cnt, PadCount());
+}
+
 static int ensure_outputs(letkf_gpu_t* h) {
   const int64_t k = h->k, ns = h->ns, nij = h->nij, G = h->G;
   if (!h->d_ana) {
@@ -538,8 +553,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
     HIPCHK(h, dalloc(&h->d_maxhz, (size_t)nij));
     HIPCHK(h, dalloc(&h->d_obscount, (size_t)nij));
     HIPCHK(h, dalloc(&h->d_mld, (size_t)nij));
-    HIPCHK(h, dalloc(&h->d_maxcount, 1));
-    HIPCHK(h, dalloc(&h->d_stats, 2));
+    HIPCHK(h, dalloc(&h->d_ctr, (size_t)kCtrSlots * kCtrStride));
   }
   // LETKF_GPU_EIGEN=jacobi selects the parallel Jacobi solver for every
   // problem (reference solver for tests); default: tridiagonal QL pipeline
@@ -588,7 +602,8 @@ static int ensure_outputs(letkf_gpu_t* h) {
     }
   }
   size_t tb = 0;
-  HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, h->d_cnt, h->d_off, (int)np, h->stream));
+  HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, padded_counts(h->d_cnt), h->d_off,
+                                             (int)np, h->stream));
   HIPCHK(h, dalloc((char**)&h->d_tmp, tb));
   h->tmp_bytes = tb;
   if (h->batched()) {
@@ -638,6 +653,9 @@ int letkf_gpu_run(letkf_gpu_t* h) {
   if (eigen_lds_bytes((int)h->k) + 8192 + 64 > 150 * 1024)
     return fail(h, LETKF_EUNSUPPORTED, "ens_size %lld too large for the FP64 eigensolver",
                 (long long)h->k);
+  if (h->batched() && h->nobs * (int64_t)h->kp * 4 >= (1ll << 31))
+    return fail(h, LETKF_EUNSUPPORTED, "batched path: n_obs x %d rows above 2^29 values",
+                h->kp);
   int rc = ensure_outputs(h);
   if (rc) return rc;
   hipStream_t st = h->stream;
@@ -646,12 +664,13 @@ int letkf_gpu_run(letkf_gpu_t* h) {
   StateDev S{h->k, h->ns, h->nij, h->d_state, h->d_mean, h->d_sprd, h->d_lat, h->d_lon,
              h->d_mask, h->d_gx, h->d_gy, h->d_gz};
   ObsDev O{h->nobs, h->d_ox, h->d_oy, h->d_oz, h->d_ozdim, h->d_oerr, h->d_odep,
-           h->d_oidx, h->d_osurf, h->d_otwp, h->d_otws, h->d_ohx, h->d_cell_start};
+           h->d_oidx, h->d_osurf, h->d_otwp, h->d_otws, h->d_ohx, h->d_cell_start,
+           h->d_ohxp, h->kp};
   VarsDev V{h->nvars, h->d_slab_var, h->d_slab_lev, h->d_inc_max, h->d_bounds, h->d_lev_depth};
   DiagDev D{h->d_maxhz, h->d_obscount, h->d_lgm, h->d_lgloc, h->d_lghash, h->d_mld,
-            h->d_maxcount, h->d_stats};
-  HIPCHK(h, hipMemsetAsync(h->d_maxcount, 0, sizeof(int32_t), st));
-  HIPCHK(h, hipMemsetAsync(h->d_stats, 0, 2 * sizeof(unsigned long long), st));
+            h->d_ctr};
+  const size_t ctr_bytes = sizeof(unsigned long long) * kCtrSlots * kCtrStride;
+  HIPCHK(h, hipMemsetAsync(h->d_ctr, 0, ctr_bytes, st));
 
   std::vector<hipEvent_t> ev;
   auto rec = [&]() -> int {
@@ -671,15 +690,20 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     launch_search(st, false, h->loc, O, S, V, C, D);
     HIPCHK(h, hipGetLastError());
     size_t tb = h->tmp_bytes;
-    HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_tmp, tb, h->d_cnt, h->d_off, (int)np, st));
+    HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_tmp, tb, padded_counts(h->d_cnt),
+                                               h->d_off, (int)np, st));
     HIPCHK(h, hipMemcpyAsync(h->h_tot, h->d_off + (np - 1), 8, hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipMemcpyAsync(h->h_tot + 1, h->d_cnt + (np - 1), 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipMemcpyAsync(h->h_tot + 2, h->d_maxcount, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(h->h_ctr, h->d_ctr, ctr_bytes, hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipStreamSynchronize(st));
-    const int64_t total = h->h_tot[0] + h->h_tot[1];
-    const int32_t maxc = (int32_t)(h->h_tot[2] & 0xffffffff);
-    if (maxc > h->dims.max_obs)
-      return fail(h, LETKF_ECAPACITY, "a gridpoint has %d observations in its search radius, "
+    // list space: padded offsets, plus one alignment unit of slack for the
+    // vector loads past the last list
+    const int64_t total = h->h_tot[0] + PadCount()(h->h_tot[1]) + kListAlign;
+    unsigned long long maxc = 0;
+    for (int sl = 0; sl < kCtrSlots; ++sl)
+      maxc = std::max(maxc, h->h_ctr[sl * kCtrStride + kCtrMax]);
+    if ((long long)maxc > (long long)h->dims.max_obs)
+      return fail(h, LETKF_ECAPACITY, "a gridpoint has %llu observations in its search radius, "
                   "above max_obs = %lld", maxc, (long long)h->dims.max_obs);
     if (total > h->list_cap) {
       const int64_t cap = total + total / 4 + 1024;
@@ -689,7 +713,6 @@ int letkf_gpu_run(letkf_gpu_t* h) {
       C.list_idx = h->d_list_idx;
       C.list_w = h->d_list_w;
     }
-    T.n_pairs += total;
     rec();
     launch_search(st, true, h->loc, O, S, V, C, D);
     HIPCHK(h, hipGetLastError());
@@ -699,8 +722,10 @@ int letkf_gpu_run(letkf_gpu_t* h) {
       size_t sb = h->stmp_bytes;
       HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(h->d_stmp, sb, h->d_bkey, h->d_bkey2,
                                                    h->d_bval, h->d_perm, (int)np, 0, 32, st));
+      rec();
       launch_product_mx(st, h->dims.precision, O, S, C, h->d_perm, h->d_B, h->d_b);
     } else {
+      rec();
       launch_product_f64(st, O, S, C, h->d_B, h->d_b);
     }
     HIPCHK(h, hipGetLastError());
@@ -726,21 +751,26 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     (void)k;
   }
   {
-    unsigned long long stt[2] = {0, 0};
-    HIPCHK(h, hipMemcpyAsync(stt, h->d_stats, sizeof stt, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(h->h_ctr, h->d_ctr, ctr_bytes, hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipStreamSynchronize(st));
-    T.n_problems = (int64_t)stt[0];
-    T.n_cand = (int64_t)stt[1];
+    for (int sl = 0; sl < kCtrSlots; ++sl) {
+      T.n_problems += (int64_t)h->h_ctr[sl * kCtrStride + kCtrSolves];
+      T.n_cand += (int64_t)h->h_ctr[sl * kCtrStride + kCtrCand];
+      T.n_pairs += (int64_t)h->h_ctr[sl * kCtrStride + kCtrPairs];
+    }
   }
-  // events: start, then per chunk [count+scan, fill, product, eigen, finalize]
-  for (size_t c = 0; c + 5 < ev.size(); c += 5) {
-    float a, b, p, e, f;
+  // events: start, then per chunk [count+scan, fill, order, product, eigen,
+  // finalize]
+  for (size_t c = 0; c + 6 < ev.size(); c += 6) {
+    float a, b, o, p, e, f;
     hipEventElapsedTime(&a, ev[c], ev[c + 1]);
     hipEventElapsedTime(&b, ev[c + 1], ev[c + 2]);
-    hipEventElapsedTime(&p, ev[c + 2], ev[c + 3]);
-    hipEventElapsedTime(&e, ev[c + 3], ev[c + 4]);
-    hipEventElapsedTime(&f, ev[c + 4], ev[c + 5]);
+    hipEventElapsedTime(&o, ev[c + 2], ev[c + 3]);
+    hipEventElapsedTime(&p, ev[c + 3], ev[c + 4]);
+    hipEventElapsedTime(&e, ev[c + 4], ev[c + 5]);
+    hipEventElapsedTime(&f, ev[c + 5], ev[c + 6]);
     T.search_ms += a + b;
+    T.order_ms += o;
     T.product_ms += p;
     T.eigen_ms += e;
     T.finalize_ms += f;
@@ -830,20 +860,23 @@ int letkf_gpu_get_selection(letkf_gpu_t* h, int64_t ij, int32_t group, int32_t* 
   StateDev S{h->k, h->ns, h->nij, h->d_state, h->d_mean, h->d_sprd, h->d_lat, h->d_lon,
              h->d_mask, h->d_gx, h->d_gy, h->d_gz};
   ObsDev O{h->nobs, h->d_ox, h->d_oy, h->d_oz, h->d_ozdim, h->d_oerr, h->d_odep,
-           h->d_oidx, h->d_osurf, h->d_otwp, h->d_otws, h->d_ohx, h->d_cell_start};
+           h->d_oidx, h->d_osurf, h->d_otwp, h->d_otws, h->d_ohx, h->d_cell_start,
+           h->d_ohxp, h->kp};
   VarsDev V{h->nvars, h->d_slab_var, h->d_slab_lev, h->d_inc_max, h->d_bounds, h->d_lev_depth};
   DiagDev D{h->d_maxhz, h->d_obscount, h->d_lgm, h->d_lgloc, h->d_lghash, h->d_mld,
-            h->d_maxcount, h->d_stats};
+            h->d_ctr};
   hipStream_t st = h->stream;
   launch_search(st, false, h->loc, O, S, V, C, D);
   size_t tb = h->tmp_bytes;
-  HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_tmp, tb, h->d_cnt, h->d_off, (int)h->G, st));
+  HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_tmp, tb, padded_counts(h->d_cnt), h->d_off,
+                                             (int)h->G, st));
   HIPCHK(h, hipMemcpyAsync(h->h_tot, h->d_off + group, 8, hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipMemcpyAsync(h->h_tot + 1, h->d_cnt + group, 8, hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipMemcpyAsync(h->h_tot + 2, h->d_off + (h->G - 1), 8, hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipMemcpyAsync(h->h_tot + 3, h->d_cnt + (h->G - 1), 8, hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipStreamSynchronize(st));
-  const int64_t off = h->h_tot[0], m = h->h_tot[1], total = h->h_tot[2] + h->h_tot[3];
+  const int64_t off = h->h_tot[0], m = h->h_tot[1];
+  const int64_t total = h->h_tot[2] + PadCount()(h->h_tot[3]) + kListAlign;
   if (total > h->list_cap) {
     const int64_t c2 = total + 1024;
     HIPCHK(h, dalloc(&h->d_list_idx, (size_t)c2));

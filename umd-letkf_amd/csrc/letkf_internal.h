@@ -47,6 +47,16 @@ struct ObsDev {
   const float* tw_sat;
   const float* hx;           // (k, n) perturbations, sorted columns
   const int32_t* cell_start; // kCells + 1
+  const float* hxp;          // (kp, n): hx rows, departure in row k, zero rows
+  int kp;                    // 32-multiple >= k + 1
+};
+
+// list offsets are multiples of kListAlign entries (aligned vector loads)
+constexpr int kListAlign = 8;
+struct PadCount {
+  __host__ __device__ int64_t operator()(int64_t c) const {
+    return (c + kListAlign - 1) & ~(int64_t)(kListAlign - 1);
+  }
 };
 
 // background state
@@ -102,9 +112,16 @@ struct DiagDev {
   double* lg_obsloc;   // (G, nij)
   uint64_t* lg_hash;   // (G, nij)
   int32_t* mld;        // (nij) mixed-layer level (LOC_OCEAN bkg_t)
-  int32_t* max_count;  // largest search set (capacity check)
-  unsigned long long* stats;  // [0] solves, [1] candidates scanned
+  // counters of the search, spread over kCtrSlots slots (slot = block index
+  // mod kCtrSlots) so that the waves' atomics do not queue on one line:
+  // slot s holds solves at ctr[s * kCtrStride + kCtrSolves], candidates,
+  // kept pairs of solves and the largest search set at the other offsets
+  unsigned long long* ctr;
 };
+
+constexpr int kCtrSlots = 64;
+constexpr int kCtrStride = 64;        // 512 B per slot
+constexpr int kCtrSolves = 0, kCtrCand = 16, kCtrPairs = 32, kCtrMax = 48;
 
 // group of slab s at a gridpoint (letkf_loc_ocean.F90:360-375 for bkg_t)
 __device__ __forceinline__ int slab_group(const LocDev& L, const VarsDev& V,
@@ -134,6 +151,8 @@ void launch_obs_flags(hipStream_t st, const LocDev& L, int64_t n,
                       const float* time, const int32_t* sat_obs, int n_sat_obs,
                       const int32_t* sat_plats, int n_sat_plats, uint8_t* surf,
                       float* tw_prof, float* tw_sat);
+void launch_obs_pad_hx(hipStream_t st, int64_t n, int k, int kp, const float* ohx,
+                       const float* odep, float* hxp);
 void launch_grid_xyz(hipStream_t st, int64_t n, const float* lat,
                      const float* lon, double* x, double* y, double* z);
 void launch_search(hipStream_t st, bool fill, const LocDev& L,

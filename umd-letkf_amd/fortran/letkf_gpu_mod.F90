@@ -76,6 +76,7 @@ MODULE letkf_gpu_mod
   TYPE, BIND(C), PUBLIC :: letkf_gpu_timing
      REAL(c_double) :: search_ms, product_ms, eigen_ms, finalize_ms, total_ms
      INTEGER(c_int64_t) :: n_problems, n_pairs, n_cand, n_launch
+     REAL(c_double) :: order_ms
   END TYPE letkf_gpu_timing
 
   PUBLIC :: letkf_gpu_version, letkf_gpu_create, letkf_gpu_destroy, letkf_gpu_last_error, &

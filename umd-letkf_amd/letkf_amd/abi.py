@@ -77,7 +77,7 @@ class Timing(C.Structure):
                 ("eigen_ms", C.c_double), ("finalize_ms", C.c_double),
                 ("total_ms", C.c_double), ("n_problems", C.c_int64),
                 ("n_pairs", C.c_int64), ("n_cand", C.c_int64),
-                ("n_launch", C.c_int64)]
+                ("n_launch", C.c_int64), ("order_ms", C.c_double)]
 
 
 EXPORTED = [
