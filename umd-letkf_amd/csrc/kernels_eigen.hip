@@ -35,19 +35,16 @@
 // flagged and solved by the Jacobi kernel instead.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
+#include "eigen_common.h"
 #include "letkf_internal.h"
 
 namespace lk {
 
 constexpr double kEps = 2.220446049250313e-16;
 
-// sum over the K lanes of a K-thread block (K = 32 or 64)
-template <int K>
-__device__ __forceinline__ double wsumK(double v) {
-#pragma unroll
-  for (int o = K / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, K);
-  return v;
-}
+
 
 // ---------------------------------------------------------------------------
 // Householder tridiagonalization, one K-lane block per problem (k < K <= 64).
@@ -280,12 +277,6 @@ __global__ __launch_bounds__(64) void k_tql(int k, int64_t np, ChunkDev C,
 // ---------------------------------------------------------------------------
 // eigenvectors and application
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double bcast(double v, int lane) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-
 template <int K>
 __global__ __launch_bounds__(64) void k_replay(ChunkDev C, const double2* __restrict__ rot,
                                                int cap_rot, const int4* __restrict__ itr,
@@ -737,6 +728,16 @@ static void tridiag_launch(hipStream_t st, unsigned nb, size_t lds, int k, const
                            const StateDev& S, const VarsDev& V, const ChunkDev& C,
                            const DiagDev& D, double* B, const double* bvec, double* tau,
                            double* dd, double* ee, double* U, int* nvec, int* status) {
+  // k >= 32 (K = 64): the register-resident reduction, rows padded to a
+  // multiple of 8 (LETKF_GPU_TRIREG=0 keeps the LDS kernel)
+  static const bool reg = [] {
+    const char* e = getenv("LETKF_GPU_TRIREG");
+    return !(e && atoi(e) == 0);
+  }();
+  if (K == 64 && reg &&
+      launch_tridiag_reg(st, nb, (k + 7) / 8 * 8, NV, k, K, L, S, V, C, D, B, bvec, tau, dd, ee,
+                         U, nvec, status))
+    return;
   k_tridiag<K, NV><<<nb, K, lds, st>>>(k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status);
 }
 

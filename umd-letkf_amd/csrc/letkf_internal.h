@@ -168,6 +168,13 @@ void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev
                        const ChunkDev& C, const int32_t* perm, double* B, double* bvec);
 void launch_bucket_keys(hipStream_t st, int mode, int64_t np, const int64_t* cnt,
                         const uint8_t* solve, uint32_t* keys, int32_t* vals);
+// register-resident tridiagonalization for rows of KQ (a multiple of 8,
+// k <= KQ <= 64); false if no kernel exists for (KQ, NV)
+bool launch_tridiag_reg(hipStream_t st, unsigned nb, int KQ, int NV, int k, int K,
+                        const LocDev& L, const StateDev& S, const VarsDev& V,
+                        const ChunkDev& C, const DiagDev& D, double* B, const double* bvec,
+                        double* tau, double* dd, double* ee, double* U, int* nvec,
+                        int* status);
 int eigen_pipeline_K(int k);
 void eigen_caps(int k, int* cap_rot, int* cap_it);
 // vectors carried per problem on the vector path (b + slabs of a group,

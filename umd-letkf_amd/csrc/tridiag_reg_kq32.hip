@@ -1,0 +1,14 @@
+// tridiag_reg_kq32.hip -- register-resident tridiagonalization, rows of 32.
+#include "tridiag_reg.h"
+
+namespace lk {
+
+bool launch_tridiag_reg_32(hipStream_t st, unsigned nb, int NV, int k, int K, const LocDev& L,
+                           const StateDev& S, const VarsDev& V, const ChunkDev& C,
+                           const DiagDev& D, double* B, const double* bvec, double* tau,
+                           double* dd, double* ee, double* U, int* nvec, int* status) {
+  return launch_tridiag_reg_kq<32>(st, nb, NV, k, K, L, S, V, C, D, B, bvec, tau, dd, ee, U,
+                                    nvec, status);
+}
+
+}  // namespace lk

@@ -37,9 +37,16 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+# per-source extra flags: the register-resident tridiagonalization needs its
+# step loop fully unrolled (static register indices), beyond LLVM's default
+# size limit for unroll pragmas
+EXTRA = {f"tridiag_reg_kq{kq}.hip": ["-mllvm", "-pragma-unroll-threshold=2000000"]
+         for kq in (32, 40, 48, 56, 64)}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [hipcc(), *FLAGS, "-c", src, "-o", obj]
+    cmd = [hipcc(), *FLAGS, *EXTRA.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
@@ -49,7 +56,7 @@ def _compile(src: str) -> str:
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
-    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+    with ThreadPoolExecutor(max_workers=min(12, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = OUT + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined",
