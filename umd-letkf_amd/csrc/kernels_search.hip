@@ -74,12 +74,24 @@ __device__ __forceinline__ bool group_has_slabs(const LocDev& L, int g, int mld)
   return true;
 }
 
-template <bool FILL>
+// Passes (one wave per gridpoint):
+//   kCountExact  kept counts and diagnostics per group (weights evaluated)
+//   kFill        lists written at the offsets of the exact counts
+//   kCountBound  one group: search-set size only (chord test, no weights),
+//                an upper bound of the kept count used to size the list
+//   kFillDiag    one group: lists written at the bound offsets, then the kept
+//                count, the solve flag and the diagnostics
+constexpr int kCountExact = 0, kFill = 1, kCountBound = 2, kFillDiag = 3;
+
+template <int PASS>
 __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
                                                 VarsDev V, ChunkDev C, DiagDev D) {
+  constexpr bool FILL = PASS == kFill || PASS == kFillDiag;
+  constexpr bool STATS = PASS == kCountExact || PASS == kFillDiag;   // sums, hash
   __shared__ int64_t s_cnt[4][kMaxGroups];
   __shared__ double s_sum[4][kMaxGroups];
   __shared__ uint64_t s_hash[4][kMaxGroups];
+  __shared__ int64_t s_lim[4][kMaxGroups];       // fill: end of each list's space
   const int wid = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t t = (int64_t)blockIdx.x * 4 + wid;
@@ -90,13 +102,14 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
   if (lane == 0) {
     for (int g = 0; g < G; ++g) {
       s_cnt[wid][g] = FILL ? C.off[t * G + g] : 0;
+      s_lim[wid][g] = FILL ? C.off[t * G + g] + C.cnt[t * G + g] : 0;
       s_sum[wid][g] = 0.0;
-      // fill pass: end of this pair's list (writes never pass it)
-      s_hash[wid][g] = FILL ? (uint64_t)(C.off[t * G + g] + C.cnt[t * G + g]) : 0;
+      s_hash[wid][g] = 0;
     }
   }
   if (S.mask[ij]) {                                   // src/letkf_solver.F90:207
-    if (!FILL && lane == 0) {
+    if (PASS == kCountExact || PASS == kCountBound) {
+      if (lane != 0) return;
       for (int g = 0; g < G; ++g) {
         C.cnt[t * G + g] = 0;
         C.solve[t * G + g] = 0;
@@ -187,6 +200,7 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
         const uint64_t imask = __ballot(inside);
         if (imask == 0) continue;
         nin += __popcll(imask);
+        if (PASS == kCountBound) continue;
         float d = 0.0f, hz = 0.0f, zd = 0.0f, twp = 1.0f, tws = 1.0f;
         uint8_t sf = 0;
         uint64_t mix = 0;
@@ -195,7 +209,7 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
           if (L.cls != LETKF_LOC_OCEAN) hz = dm::gc(d, Lh);
           if (L.cls == LETKF_LOC_OCEAN) { sf = O.surf[s]; twp = O.tw_prof[s]; tws = O.tw_sat[s]; }
           if (L.cls == LETKF_LOC_VGAUSS) zd = O.zdim[s];
-          if (!FILL) mix = dm::mix64((uint64_t)O.oidx[s]);
+          if (STATS) mix = dm::mix64((uint64_t)O.oidx[s]);
         }
         for (int g = 0; g < G; ++g) {
           float w = 0.0f;
@@ -203,17 +217,19 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
           const bool keep = inside && (w > 1.0e-4f);   // src/letkf_solver.F90:251
           const uint64_t kmask = __ballot(keep);
           if (kmask == 0) continue;
-          if (!FILL) {
+          if (STATS) {
             const double sr = wave_sum_f64(keep ? (double)w : 0.0);
             const uint64_t hs = wave_sum_u64(keep ? mix : 0ull);
             if (lane == 0) {
-              s_cnt[wid][g] += __popcll(kmask);
+              if (!FILL) s_cnt[wid][g] += __popcll(kmask);
               s_sum[wid][g] += sr;
               s_hash[wid][g] += hs;
             }
-          } else {
+          }
+          if (FILL) {
+            // writes never pass the end of this pair's list space
             int64_t cur = 0, lim = 0;
-            if (lane == 0) { cur = s_cnt[wid][g]; lim = (int64_t)s_hash[wid][g]; }
+            if (lane == 0) { cur = s_cnt[wid][g]; lim = s_lim[wid][g]; }
             cur = __shfl(cur, 0, 64);
             lim = __shfl(lim, 0, 64);
             if (keep) {
@@ -230,26 +246,31 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     }
   }
 
-  if (!FILL && lane == 0) {
+  if (lane != 0 || PASS == kFill) return;
+  unsigned long long* c = D.ctr + (blockIdx.x & (kCtrSlots - 1)) * kCtrStride;
+  if (PASS == kCountExact || PASS == kFillDiag) {
     unsigned long long nsolve = 0, npair = 0;
     for (int g = 0; g < G; ++g) {
-      const int64_t m = s_cnt[wid][g];
-      nsolve += (m > 0 && group_has_slabs(L, g, mld)) ? 1 : 0;
-      npair += (m > 0 && group_has_slabs(L, g, mld)) ? (unsigned long long)m : 0ull;
+      const int64_t m = PASS == kFillDiag ? s_cnt[wid][g] - C.off[t * G + g] : s_cnt[wid][g];
+      const bool sv = m > 0 && group_has_slabs(L, g, mld);
+      nsolve += sv ? 1 : 0;
+      npair += sv ? (unsigned long long)m : 0ull;
       C.cnt[t * G + g] = m;
-      C.solve[t * G + g] = (m > 0 && group_has_slabs(L, g, mld)) ? 1 : 0;
+      C.solve[t * G + g] = sv ? 1 : 0;
       D.lg_m[ij * G + g] = (int32_t)m;
       D.lg_obsloc[ij * G + g] = s_sum[wid][g];
       D.lg_hash[ij * G + g] = s_hash[wid][g];
     }
+    atomicAdd(c + kCtrSolves, nsolve);
+    atomicAdd(c + kCtrPairs, npair);
+  }
+  if (PASS == kCountBound) C.cnt[t * G] = nin;      // one group
+  if (PASS == kCountExact || PASS == kCountBound) {
     D.obscount[ij] = (int32_t)nin;
     D.maxhz[ij] = maxhz;
     D.mld[ij] = mld;
-    unsigned long long* c = D.ctr + (blockIdx.x & (kCtrSlots - 1)) * kCtrStride;
     atomicMax(c + kCtrMax, (unsigned long long)nin);
-    atomicAdd(c + kCtrSolves, nsolve);
     atomicAdd(c + kCtrCand, (unsigned long long)ncand);
-    atomicAdd(c + kCtrPairs, npair);
   }
 }
 
@@ -258,8 +279,15 @@ void launch_search(hipStream_t st, bool fill, const LocDev& L, const ObsDev& O,
                    const DiagDev& D) {
   if (C.ngp <= 0) return;
   const unsigned nb = (unsigned)((C.ngp + 3) / 4);
-  if (fill) k_search<true><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
-  else k_search<false><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+  // one group: size the lists by the search set (chord test only) and count
+  // the kept observations while writing them
+  if (C.G == 1) {
+    if (fill) k_search<kFillDiag><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+    else k_search<kCountBound><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+  } else {
+    if (fill) k_search<kFill><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+    else k_search<kCountExact><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+  }
 }
 
 }  // namespace lk

@@ -875,7 +875,7 @@ int letkf_gpu_get_selection(letkf_gpu_t* h, int64_t ij, int32_t group, int32_t* 
   HIPCHK(h, hipMemcpyAsync(h->h_tot + 2, h->d_off + (h->G - 1), 8, hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipMemcpyAsync(h->h_tot + 3, h->d_cnt + (h->G - 1), 8, hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipStreamSynchronize(st));
-  const int64_t off = h->h_tot[0], m = h->h_tot[1];
+  const int64_t off = h->h_tot[0];
   const int64_t total = h->h_tot[2] + PadCount()(h->h_tot[3]) + kListAlign;
   if (total > h->list_cap) {
     const int64_t c2 = total + 1024;
@@ -887,7 +887,10 @@ int letkf_gpu_get_selection(letkf_gpu_t* h, int64_t ij, int32_t group, int32_t* 
   }
   launch_search(st, true, h->loc, O, S, V, C, D);
   HIPCHK(h, hipGetLastError());
+  // kept count (with one group the count pass gives only a bound)
+  HIPCHK(h, hipMemcpyAsync(h->h_tot + 1, h->d_cnt + group, 8, hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipStreamSynchronize(st));
+  const int64_t m = h->h_tot[1];
   *m_out = m;
   const int64_t nc = std::min(m, cap);
   if (nc > 0) {

@@ -53,11 +53,25 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = True) -> str:
+def _stale(src: str, newest_header: float) -> bool:
+    obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
+    return (not os.path.exists(obj) or
+            os.path.getmtime(obj) < max(os.path.getmtime(src), newest_header))
+
+
+def build(verbose: bool = True, force: bool = True) -> str:
+    """Compile and link the library; force=False recompiles only sources
+    older than their object or than the newest header."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
-    with ThreadPoolExecutor(max_workers=min(12, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+    headers = glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(
+        os.path.join(ROOT, "..", "include", "*.h"))
+    newest = max((os.path.getmtime(h) for h in headers), default=0.0)
+    todo = [s for s in srcs if force or _stale(s, newest)]
+    if todo:
+        with ThreadPoolExecutor(max_workers=min(12, len(todo))) as ex:
+            list(ex.map(_compile, todo))
+    objs = [os.path.join(BUILD, os.path.basename(s).replace(".hip", ".o")) for s in srcs]
     tmp = OUT + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined",
            "-o", tmp, *objs]
