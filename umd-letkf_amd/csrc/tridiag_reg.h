@@ -12,10 +12,12 @@ namespace lk {
 
 // ---------------------------------------------------------------------------
 // The same reduction with the matrix in registers: lane r keeps row r of A
-// (a[c] = A(r, c), KQ >= k a multiple of 8) and only the Householder vector v
-// and w = t A v - (t^2 v.Av / 2) v go through LDS, as (v_c, w_c) pairs read
-// by all lanes.  The step loop is unrolled so that every register index is
-// static; columns c <= j are no longer touched at step j.
+// (a[c] = A(r, c), KQ >= k a multiple of 8).  The Householder vector v goes
+// through LDS for the symmetric product A v (read by all lanes), and the
+// rank-2 update takes v_c and w_c = (t A v - (t^2 v.Av / 2) v)_c from lane c
+// by readlane, so the LDS and the VALU share the broadcast work.  The step
+// loop is unrolled so that every register index is static; columns c <= j
+// are no longer touched at step j.
 // ---------------------------------------------------------------------------
 template <int KQ, int NV>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
@@ -25,7 +27,6 @@ void k_tridiag_reg(int k, int K, LocDev L, StateDev S, VarsDev V, ChunkDev C, Di
                    double* __restrict__ ee, double* __restrict__ U,
                    int* __restrict__ nvec, int* __restrict__ status) {
   __shared__ double sv[KQ];
-  __shared__ double2 svw[KQ];
   __shared__ int sl[NV > 0 ? NV : 1];
   const int64_t pair = blockIdx.x;
   if (!C.solve[pair]) return;
@@ -100,14 +101,12 @@ void k_tridiag_reg(int k, int K, LocDev L, StateDev S, VarsDev V, ChunkDev C, Di
         const double p = (r > j && r < k) ? t * (p0 + p1) : 0.0;
         const double kc = 0.5 * t * wsumK<64>(p * v);
         const double w = p - kc * v;
-        if (r < KQ) svw[r] = make_double2(v, w);
-        __syncthreads();
+        // rank-2 update; v_c and w_c come from lane c by readlane (VALU),
+        // which leaves the LDS to the symv above
 #pragma unroll
         for (int c = j + 1; c < KQ; ++c) {
-          const double2 q = svw[c];
-          a[c] = fma(-w, q.x, fma(-v, q.y, a[c]));
-          // groups of 4: each update is pinned before the next group's LDS
-          // reads are issued (bounds the registers held by reads in flight)
+          const double vc = bcast(v, c), wc = bcast(w, c);
+          a[c] = fma(-w, vc, fma(-v, wc, a[c]));
           if ((c & 3) == 3) asm volatile("" : "+v"(a[c]) :: "memory");
           else asm volatile("" : "+v"(a[c]));
         }
