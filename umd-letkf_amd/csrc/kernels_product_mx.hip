@@ -44,38 +44,43 @@ __device__ __forceinline__ int obs_slot(int half, int e) {
 // followed by at least 8 readable entries, and the padded columns hxp carry
 // the departure in row k and zeros above it.  Entries past m read column 0
 // with weight 0.
+// Operand registers of one 16-observation step.  The kept list of a problem
+// holds (column byte offset, weight) pairs from a multiple of 8 entries on,
+// zero-padded to the next multiple of 8; both are read through buffer
+// descriptors whose range ends there, so entries past the list read as
+// (0, 0): column 0 with weight 0.  The padded columns carry the departure in
+// row k and zeros above it, so no row or range test is needed.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 template <int NB, int PREC>
-__device__ __forceinline__ void load_frag(MxFrag<NB>& F, const ObsDev& O,
-                                          __amdgpu_buffer_rsrc_t hxr, const ChunkDev& C,
-                                          int64_t off, int64_t m, int64_t j0, int lane) {
+__device__ __forceinline__ void load_frag(MxFrag<NB>& F, __amdgpu_buffer_rsrc_t hxr,
+                                          __amdgpu_buffer_rsrc_t ir, __amdgpu_buffer_rsrc_t wr,
+                                          int j0, int lane) {
   const int half = lane >> 5, i = lane & 31;
-  int32_t sidx[8];
-  float w[8];
+  uint32_t sidx[8], wb[8];
   if (PREC == LETKF_PREC_FP32) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const int64_t j = off + j0 + obs_slot<PREC>(half, e);
-      sidx[e] = C.list_idx[j];
-      w[e] = C.list_w[j];                 // sqrt(rloc / err), written by the search
+      const uint32_t vo = (uint32_t)(j0 + obs_slot<PREC>(half, e)) * 4u;
+      sidx[e] = __builtin_amdgcn_raw_buffer_load_b32(ir, vo, 0, 0);
+      wb[e] = __builtin_amdgcn_raw_buffer_load_b32(wr, vo, 0, 0);
     }
   } else {
-    const int4* pi = reinterpret_cast<const int4*>(C.list_idx + off + j0 + 8 * half);
-    const float4* pw = reinterpret_cast<const float4*>(C.list_w + off + j0 + 8 * half);
-    const int4 a0 = pi[0], a1 = pi[1];
-    const float4 b0 = pw[0], b1 = pw[1];
-    sidx[0] = a0.x; sidx[1] = a0.y; sidx[2] = a0.z; sidx[3] = a0.w;
-    sidx[4] = a1.x; sidx[5] = a1.y; sidx[6] = a1.z; sidx[7] = a1.w;
-    w[0] = b0.x; w[1] = b0.y; w[2] = b0.z; w[3] = b0.w;
-    w[4] = b1.x; w[5] = b1.y; w[6] = b1.z; w[7] = b1.w;
+    const uint32_t vo = (uint32_t)(j0 + 8 * half) * 4u;
+    const u32x4 a0 = __builtin_amdgcn_raw_buffer_load_b128(ir, vo, 0, 0);
+    const u32x4 a1 = __builtin_amdgcn_raw_buffer_load_b128(ir, vo, 16, 0);
+    const u32x4 b0 = __builtin_amdgcn_raw_buffer_load_b128(wr, vo, 0, 0);
+    const u32x4 b1 = __builtin_amdgcn_raw_buffer_load_b128(wr, vo, 16, 0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      sidx[e] = a0[e]; sidx[4 + e] = a1[e];
+      wb[e] = b0[e]; wb[4 + e] = b1[e];
+    }
   }
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    const bool valid = j0 + obs_slot<PREC>(half, e) < m;
-    const float ww = valid ? w[e] : 0.0f;
-    // byte offset of the lane's row in the column; past the buffer's range
-    // (returns 0) for entries beyond m
-    const uint32_t vo = valid ? ((uint32_t)sidx[e] * (uint32_t)O.kp + (uint32_t)i) * 4u
-                              : 0x80000000u;
+    const uint32_t vo = sidx[e] + 4u * (uint32_t)i;
+    const float ww = __uint_as_float(wb[e]);
 #pragma unroll
     for (int rb = 0; rb < NB; ++rb)
       F.y[rb][e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(hxr, vo, 128 * rb, 0)) * ww;
@@ -165,27 +170,32 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
   constexpr int NBLK = NB * (NB + 1) / 2;
   const int64_t pair = perm[C.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x];
   if (!C.solve[pair]) return;
-  const int64_t m = C.cnt[pair];
+  const int m = (int)C.cnt[pair];
   const int64_t off = C.off[pair];
   const int lane = threadIdx.x;
-  // padded operand columns; the host guarantees n * kp * 4 < 2^31
+  // padded operand columns (the host guarantees n * kp * 4 < 2^31) and the
+  // problem's list, ranged to its zero-padded length
   const __amdgpu_buffer_rsrc_t hxr = __builtin_amdgcn_make_buffer_rsrc(
       (void*)O.hxp, (short)0, (int)(O.n * O.kp * 4), 0x00020000);
+  const int lbytes = ((m + kListAlign - 1) & ~(kListAlign - 1)) * 4;
+  const __amdgpu_buffer_rsrc_t ir =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(C.list_idx + off), (short)0, lbytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t wr =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(C.list_w + off), (short)0, lbytes, 0x00020000);
 
   f32x16 acc[NBLK];
 #pragma unroll
   for (int b = 0; b < NBLK; ++b) acc[b] = (f32x16){};
 
   // two fragment buffers in turn: the next step's loads are in flight while
-  // the current step's MFMAs run
+  // the current step's MFMAs run; steps past m load zeros and add nothing
   MxFrag<NB> fa, fb;
-  load_frag<NB, PREC>(fa, O, hxr, C, off, m, 0, lane);
-  for (int64_t j0 = 0; j0 < m; j0 += 32) {
-    const bool second = j0 + 16 < m;
-    if (second) load_frag<NB, PREC>(fb, O, hxr, C, off, m, j0 + 16, lane);
+  load_frag<NB, PREC>(fa, hxr, ir, wr, 0, lane);
+  for (int j0 = 0; j0 < m; j0 += 32) {
+    load_frag<NB, PREC>(fb, hxr, ir, wr, j0 + 16, lane);
     mx_step<NB, PREC>(acc, fa);
-    if (j0 + 32 < m) load_frag<NB, PREC>(fa, O, hxr, C, off, m, j0 + 32, lane);
-    if (second) mx_step<NB, PREC>(acc, fb);
+    load_frag<NB, PREC>(fa, hxr, ir, wr, j0 + 32, lane);
+    mx_step<NB, PREC>(acc, fb);
   }
 
   // 32x32 f32 accumulator: register r of lane l holds row 8(r/4) + 4(l/32) + r%4
@@ -240,28 +250,53 @@ void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev
 
 // order of the problems of a chunk: mode 0 gridpoint order, mode 1 longest
 // first (descending m), mode 2 power-of-two buckets of m, largest first,
-// gridpoint order inside a bucket
-__global__ void k_bucket_keys(int mode, int64_t np, const int64_t* __restrict__ cnt,
-                              const uint8_t* __restrict__ solve, uint32_t* keys,
-                              int32_t* vals) {
+// gridpoint order inside a bucket, mode 3 the same buckets with a Morton
+// (Z-order) walk over a 0.25-degree lat/lon lattice inside a bucket, so that
+// problems in flight together share observation columns in L2
+__device__ __forceinline__ uint32_t spread_bits(uint32_t x) {   // 16 -> 32 bits
+  x &= 0xffffu;
+  x = (x | (x << 8)) & 0x00ff00ffu;
+  x = (x | (x << 4)) & 0x0f0f0f0fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+
+__global__ void k_bucket_keys(int mode, int64_t np, int G, int64_t i0,
+                              const int64_t* __restrict__ cnt,
+                              const uint8_t* __restrict__ solve,
+                              const float* __restrict__ lat, const float* __restrict__ lon,
+                              uint32_t* keys, int32_t* vals) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const int64_t m = solve[p] ? cnt[p] : 0;
   uint32_t key = (uint32_t)p;
+  const uint32_t b = m > 0 ? 64u - (uint32_t)__clzll((unsigned long long)m) : 0u;
+  const uint32_t bucket = 31u - (b > 31u ? 31u : b);
   if (mode == 1) {
     key = 0xffffffffu - (uint32_t)(m > 0xfffffffe ? 0xfffffffe : m);
   } else if (mode == 2) {
-    const uint32_t b = m > 0 ? 64u - (uint32_t)__clzll((unsigned long long)m) : 0u;
-    key = ((31u - (b > 31u ? 31u : b)) << 27) | (uint32_t)p;
+    key = (bucket << 27) | (uint32_t)p;
+  } else if (mode == 3) {
+    const int64_t ij = i0 + p / G;
+    float fl = (lat[ij] + 90.0f) * 4.0f;
+    float fo = lon[ij] - 360.0f * floorf(lon[ij] / 360.0f);
+    fo *= 4.0f;
+    const uint32_t li = (uint32_t)fminf(fmaxf(fl, 0.0f), 1023.0f);
+    const uint32_t lo = (uint32_t)fminf(fmaxf(fo, 0.0f), 2047.0f);
+    const uint32_t z = spread_bits(li) | (spread_bits(lo) << 1);   // 22 bits
+    key = (bucket << 27) | (z << 5) | (uint32_t)(p % G & 31);
   }
   keys[p] = key;
   vals[p] = (int32_t)p;
 }
 
-void launch_bucket_keys(hipStream_t st, int mode, int64_t np, const int64_t* cnt,
-                        const uint8_t* solve, uint32_t* keys, int32_t* vals) {
+void launch_bucket_keys(hipStream_t st, int mode, int64_t np, int G, int64_t i0,
+                        const int64_t* cnt, const uint8_t* solve, const float* lat,
+                        const float* lon, uint32_t* keys, int32_t* vals) {
   if (np <= 0) return;
-  k_bucket_keys<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(mode, np, cnt, solve, keys, vals);
+  k_bucket_keys<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(mode, np, G, i0, cnt, solve, lat,
+                                                             lon, keys, vals);
 }
 
 }  // namespace lk

@@ -235,7 +235,8 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
             if (keep) {
               const int64_t pos = cur + lane_rank(kmask);
               if (pos < lim) {
-                C.list_idx[pos] = (int32_t)s;
+                // mixed products: byte offset of the padded operand column
+                C.list_idx[pos] = C.list_mode ? (int32_t)(s * O.kp * 4) : (int32_t)s;
                 C.list_w[pos] = C.list_mode ? sqrtf(w / O.err[s]) : w;
               }
             }
@@ -246,6 +247,20 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     }
   }
 
+  if (FILL && C.list_mode) {
+    // zero the alignment padding after each list: the mixed products read
+    // whole 8-entry groups
+    for (int g = 0; g < G; ++g) {
+      int64_t end = 0;
+      if (lane == 0) end = s_cnt[wid][g];
+      end = __shfl(end, 0, 64);
+      const int64_t pad_end = C.off[t * G + g] + PadCount()(end - C.off[t * G + g]);
+      if (end + lane < pad_end) {
+        C.list_idx[end + lane] = 0;
+        C.list_w[end + lane] = 0.0f;
+      }
+    }
+  }
   if (lane != 0 || PASS == kFill) return;
   unsigned long long* c = D.ctr + (blockIdx.x & (kCtrSlots - 1)) * kCtrStride;
   if (PASS == kCountExact || PASS == kFillDiag) {

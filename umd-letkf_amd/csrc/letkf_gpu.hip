@@ -718,7 +718,8 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     HIPCHK(h, hipGetLastError());
     rec();
     if (h->batched()) {
-      launch_bucket_keys(st, h->tune_bucket, np, h->d_cnt, h->d_solve, h->d_bkey, h->d_bval);
+      launch_bucket_keys(st, h->tune_bucket, np, (int)G, i0, h->d_cnt, h->d_solve, h->d_lat,
+                         h->d_lon, h->d_bkey, h->d_bval);
       size_t sb = h->stmp_bytes;
       HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(h->d_stmp, sb, h->d_bkey, h->d_bkey2,
                                                    h->d_bval, h->d_perm, (int)np, 0, 32, st));

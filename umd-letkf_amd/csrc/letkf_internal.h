@@ -166,8 +166,9 @@ void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
                             const double* bvec, const int* only_flagged, double* ana);
 void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev& S,
                        const ChunkDev& C, const int32_t* perm, double* B, double* bvec);
-void launch_bucket_keys(hipStream_t st, int mode, int64_t np, const int64_t* cnt,
-                        const uint8_t* solve, uint32_t* keys, int32_t* vals);
+void launch_bucket_keys(hipStream_t st, int mode, int64_t np, int G, int64_t i0,
+                        const int64_t* cnt, const uint8_t* solve, const float* lat,
+                        const float* lon, uint32_t* keys, int32_t* vals);
 // register-resident tridiagonalization for rows of KQ (a multiple of 8,
 // k <= KQ <= 64); false if no kernel exists for (KQ, NV)
 bool launch_tridiag_reg(hipStream_t st, unsigned nb, int KQ, int NV, int k, int K,
