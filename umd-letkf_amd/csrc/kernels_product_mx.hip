@@ -52,18 +52,21 @@ __device__ __forceinline__ int obs_slot(int half, int e) {
 // row k and zeros above it, so no row or range test is needed.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-template <int NB, int PREC>
-__device__ __forceinline__ void load_frag(MxFrag<NB>& F, __amdgpu_buffer_rsrc_t hxr,
-                                          __amdgpu_buffer_rsrc_t ir, __amdgpu_buffer_rsrc_t wr,
-                                          int j0, int lane) {
-  const int half = lane >> 5, i = lane & 31;
+// (column byte offset, weight bits) of the lane's 8 observations of a step
+struct ListRegs {
   uint32_t sidx[8], wb[8];
+};
+
+template <int PREC>
+__device__ __forceinline__ void load_list(ListRegs& Lr, __amdgpu_buffer_rsrc_t ir,
+                                          __amdgpu_buffer_rsrc_t wr, int j0, int lane) {
+  const int half = lane >> 5;
   if (PREC == LETKF_PREC_FP32) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const uint32_t vo = (uint32_t)(j0 + obs_slot<PREC>(half, e)) * 4u;
-      sidx[e] = __builtin_amdgcn_raw_buffer_load_b32(ir, vo, 0, 0);
-      wb[e] = __builtin_amdgcn_raw_buffer_load_b32(wr, vo, 0, 0);
+      Lr.sidx[e] = __builtin_amdgcn_raw_buffer_load_b32(ir, vo, 0, 0);
+      Lr.wb[e] = __builtin_amdgcn_raw_buffer_load_b32(wr, vo, 0, 0);
     }
   } else {
     const uint32_t vo = (uint32_t)(j0 + 8 * half) * 4u;
@@ -73,14 +76,20 @@ __device__ __forceinline__ void load_frag(MxFrag<NB>& F, __amdgpu_buffer_rsrc_t 
     const u32x4 b1 = __builtin_amdgcn_raw_buffer_load_b128(wr, vo, 16, 0);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      sidx[e] = a0[e]; sidx[4 + e] = a1[e];
-      wb[e] = b0[e]; wb[4 + e] = b1[e];
+      Lr.sidx[e] = a0[e]; Lr.sidx[4 + e] = a1[e];
+      Lr.wb[e] = b0[e]; Lr.wb[4 + e] = b1[e];
     }
   }
+}
+
+template <int NB>
+__device__ __forceinline__ void load_cols(MxFrag<NB>& F, __amdgpu_buffer_rsrc_t hxr,
+                                          const ListRegs& Lr, int lane) {
+  const int i = lane & 31;
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    const uint32_t vo = sidx[e] + 4u * (uint32_t)i;
-    const float ww = __uint_as_float(wb[e]);
+    const uint32_t vo = Lr.sidx[e] + 4u * (uint32_t)i;
+    const float ww = __uint_as_float(Lr.wb[e]);
 #pragma unroll
     for (int rb = 0; rb < NB; ++rb)
       F.y[rb][e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(hxr, vo, 128 * rb, 0)) * ww;
@@ -187,14 +196,21 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
 #pragma unroll
   for (int b = 0; b < NBLK; ++b) acc[b] = (f32x16){};
 
-  // two fragment buffers in turn: the next step's loads are in flight while
-  // the current step's MFMAs run; steps past m load zeros and add nothing
+  // software pipeline: the list entries of a step are loaded two steps
+  // ahead, its columns one step ahead of its MFMAs; steps past m load zeros
+  // and add nothing
   MxFrag<NB> fa, fb;
-  load_frag<NB, PREC>(fa, hxr, ir, wr, 0, lane);
+  ListRegs la, lb;
+  load_list<PREC>(la, ir, wr, 0, lane);
+  load_list<PREC>(lb, ir, wr, 16, lane);
+  load_cols<NB>(fa, hxr, la, lane);
+  load_list<PREC>(la, ir, wr, 32, lane);
   for (int j0 = 0; j0 < m; j0 += 32) {
-    load_frag<NB, PREC>(fb, hxr, ir, wr, j0 + 16, lane);
+    load_cols<NB>(fb, hxr, lb, lane);
+    load_list<PREC>(lb, ir, wr, j0 + 48, lane);
     mx_step<NB, PREC>(acc, fa);
-    load_frag<NB, PREC>(fa, hxr, ir, wr, j0 + 32, lane);
+    load_cols<NB>(fa, hxr, la, lane);
+    load_list<PREC>(la, ir, wr, j0 + 64, lane);
     mx_step<NB, PREC>(acc, fb);
   }
 

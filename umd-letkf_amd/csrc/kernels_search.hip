@@ -186,6 +186,8 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
   }
 
   int64_t nin = 0, ncand = 0;
+  double lsum = 0.0;          // kFillDiag: this lane's weights and hashes
+  uint64_t lhash = 0;
   for (int b = b0; b <= b1; ++b) {
     for (int r = 0; r < nr; ++r) {
       const int64_t sb = O.cell_start[b * kLonCells + rc0[r]];
@@ -217,7 +219,11 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
           const bool keep = inside && (w > 1.0e-4f);   // src/letkf_solver.F90:251
           const uint64_t kmask = __ballot(keep);
           if (kmask == 0) continue;
-          if (STATS) {
+          if (PASS == kFillDiag) {
+            // one group: per-lane partial sums, reduced once per gridpoint
+            lsum += keep ? (double)w : 0.0;
+            lhash += keep ? mix : 0ull;
+          } else if (STATS) {
             const double sr = wave_sum_f64(keep ? (double)w : 0.0);
             const uint64_t hs = wave_sum_u64(keep ? mix : 0ull);
             if (lane == 0) {
@@ -247,6 +253,14 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     }
   }
 
+  if (PASS == kFillDiag) {
+    const double sr = wave_sum_f64(lsum);
+    const uint64_t hs = wave_sum_u64(lhash);
+    if (lane == 0) {
+      s_sum[wid][0] = sr;
+      s_hash[wid][0] = hs;
+    }
+  }
   if (FILL && C.list_mode) {
     // zero the alignment padding after each list: the mixed products read
     // whole 8-entry groups

@@ -12,6 +12,8 @@
 //                   (src/letkf_solver.F90:299-379)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "letkf_internal.h"
 
 namespace lk {
@@ -169,9 +171,13 @@ __global__ __launch_bounds__(256) void k_eigen_apply(LocDev L, StateDev S,
   __shared__ int s_nsl;
   __shared__ int s_sl[2048];
 
-  const int64_t pair = blockIdx.x;
-  if (!C.solve[pair]) return;
-  if (only_flagged && only_flagged[pair] == 0) return;
+  // one block per problem, or, when only flagged problems are solved, a
+  // bounded grid that strides over the chunk (every thread of a block walks
+  // the same problems, so the barriers below stay uniform)
+  const int64_t npair = C.ngp * C.G;
+  for (int64_t pair = blockIdx.x; pair < npair; pair += gridDim.x) {
+  if (!C.solve[pair]) continue;
+  if (only_flagged && only_flagged[pair] == 0) continue;
   const int tid = threadIdx.x;
   const int64_t t = pair / C.G;
   const int g = (int)(pair - t * C.G);
@@ -290,6 +296,8 @@ __global__ __launch_bounds__(256) void k_eigen_apply(LocDev L, StateDev S,
     for (int i = 0; i < k; ++i) acc += (double)xb[i] * A[i * lda + j];
     xa0[(int64_t)k * sl + j] = acc;
   }
+  __syncthreads();                            // LDS reused by the next problem
+  }
 }
 
 size_t eigen_lds_bytes(int k) {
@@ -310,7 +318,8 @@ void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
     attr = true;
   }
-  k_eigen_apply<<<(unsigned)np, 256, lds, st>>>(L, S, V, C, D, B, bvec, only_flagged, ana);
+  const unsigned nb = only_flagged ? (unsigned)std::min<int64_t>(np, 1024) : (unsigned)np;
+  k_eigen_apply<<<nb, 256, lds, st>>>(L, S, V, C, D, B, bvec, only_flagged, ana);
 }
 
 // ---------------------------------------------------------------------------
