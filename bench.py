@@ -167,6 +167,8 @@ def main():
                     help="skip timing the plain FP64 path on the same workload")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ingest", action="store_true",
+                    help="skip timing the file-order / raw-obs upload and download")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="record the product kernel's per-dispatch HBM traffic from two "
                          "rocprofv3 --pmc counter_collection.csv files in profiles/traffic.json")
@@ -238,6 +240,30 @@ def main():
         out["cpu_baseline"], out["error_vs_fp64_oracle"] = cpu_baseline(
             prob, solver, args.cpu_seconds, nt)
     solver.final()
+    if not args.no_ingest:
+        # the steps either side of the path on the device: members in file
+        # order -> perturbations / mean / spread, members' hx + qc -> hx
+        # mean / perturbations / compaction (5% of the obs flagged bad)
+        g = np.random.default_rng(99)
+        mem = np.asfortranarray((prob.state + prob.mean[None]).astype(np.float32).transpose(2, 1, 0))
+        hxm = np.asfortranarray((prob.hx + prob.hxmean[None]).astype(np.float32))
+        qc = (g.random(prob.nobs) < 0.05).astype(np.int32)
+        tu = time.perf_counter()
+        s2 = LetkfGpuSolver(prob, precision=args.precision, path=args.path, members=mem,
+                            obs_members=(qc, hxm))
+        up2 = time.perf_counter() - tu
+        s2.run()
+        tr = time.perf_counter()
+        s2.state_members()
+        down = time.perf_counter() - tr
+        s2.final()
+        out["ingest"] = {"upload_members_s": round(up2, 3), "upload_s": out["upload_s"],
+                         "egress_members_s": round(down, 3),
+                         "note": "host wall time incl. PCIe: set_state_members + "
+                                 "set_obs_members (device-side mean/perturbation/spread, "
+                                 "hx split, QC compaction of 5% bad obs) vs set_state + "
+                                 "set_obs; get_state_members (file-order egress)"}
+        log(f"[bench] ingest {out['ingest']}")
     if not args.no_fp64_compare and not (args.path == "plain" and args.precision == "fp64"):
         ref = LetkfGpuSolver(prob, precision="fp64", path="plain")
         for _ in range(args.warmup):

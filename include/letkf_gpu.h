@@ -173,6 +173,31 @@ int letkf_gpu_set_obs(letkf_gpu_t* h, int64_t n_obs,
                       const int32_t* obsid, const int32_t* platid,
                       const float* hx_pert, const float* hx_mean);
 
+/* Background ensemble as the members' full values in file order, members
+ * (ij_count, grid_ns, ens_size) with the gridpoint fastest: the state that
+ * letkf_state_read_ens scatters (src/letkf_state.F90:312-314) before
+ * letkf_state_init removes the mean.  The mean SUM(.,1)/ens_size, the
+ * perturbations and the spread SQRT(DOT_PRODUCT(x',x')/(ens_size-1)) are
+ * formed on the device in FP32 with the reference's order of operations
+ * (src/letkf_state.F90:326-334).  Replaces letkf_gpu_set_state's host-side
+ * preparation; ens_size must be at least 2. */
+int letkf_gpu_set_state_members(letkf_gpu_t* h, const float* members,
+                                const float* lat_ij, const float* lon_ij,
+                                const int32_t* mask_ij);
+
+/* Observations before letkf_obs_read's own processing
+ * (src/letkf_obs.F90:396-507): every obs_def field including qc, and the
+ * members' hx (ens_size, n_obs) as read by read_hx.  On the device:
+ * obs_hx_mean = SUM(hx,1)/ens_size and obs_hx = hx - obs_hx_mean in FP32
+ * (:459-466), then the obs with qc > 0 are dropped keeping the order
+ * (:479-489), then the search structure is built as in letkf_gpu_set_obs. */
+int letkf_gpu_set_obs_members(letkf_gpu_t* h, int64_t n_obs,
+                              const float* lat, const float* lon,
+                              const float* zdim, const float* time,
+                              const float* val, const float* err,
+                              const int32_t* obsid, const int32_t* platid,
+                              const int32_t* qc, const float* hx_members);
+
 /* State variables (statedef, src/letkf_state.F90:83-98): slab start
  * (0-based grid_s_idx), levels, ana_inc_max and ana_bounds (2 per var). */
 int letkf_gpu_set_statevars(letkf_gpu_t* h, int32_t nvars,
@@ -196,6 +221,13 @@ int letkf_gpu_run(letkf_gpu_t* h);
  * Masked points hold the background perturbations, mean and spread. */
 int letkf_gpu_get_state(letkf_gpu_t* h, float* state_ij, float* mean_ij,
                         float* sprd_ij);
+
+/* Analysis in file order, the layout letkf_state_write_ens writes from
+ * (src/letkf_state.F90:877-955): members (ij_count, grid_ns, ens_size),
+ * mean and sprd (ij_count, grid_ns) for letkf_state_write_meansprd
+ * (:357-451); FP32.  Any pointer may be NULL. */
+int letkf_gpu_get_state_members(letkf_gpu_t* h, float* members, float* mean_ij,
+                                float* sprd_ij);
 
 /* The same analysis at the precision it was computed in (FP64 for
  * LETKF_PREC_FP64; the FP32 results widened otherwise). */

@@ -242,3 +242,43 @@ def selection(prob, ij, group=0):
     if m < 0:
         raise RuntimeError("oracle selection failed")
     return idx[:m].copy(), rloc[:m].copy(), dist[:m].copy()
+
+
+# --------------------------------------------------------------------------
+# the steps either side of the path, FP32 in the reference's order
+# --------------------------------------------------------------------------
+def state_from_members(members):
+    """letkf_state_init's mean, perturbations and spread
+    (src/letkf_state.F90:326-334) in FP32: members (nij, ns, k) in file order
+    (gridpoint fastest), summed member by member.  Returns state (k, ns, nij)
+    perturbations, mean and sprd (ns, nij)."""
+    members = np.asarray(members, np.float32)
+    k = members.shape[2]
+    acc = np.zeros(members.shape[:2], np.float32)
+    for m in range(k):
+        acc = acc + members[:, :, m]
+    mu = acc / np.float32(k)
+    pert = members - mu[:, :, None]
+    dot = np.zeros_like(acc)
+    for m in range(k):
+        dot = dot + pert[:, :, m] * pert[:, :, m]
+    sprd = np.sqrt(dot / np.float32(k - 1))
+    # column-major like the reference's arrays (member fastest)
+    return (np.asfortranarray(pert.transpose(2, 1, 0), np.float32),
+            np.asfortranarray(mu.T), np.asfortranarray(sprd.T))
+
+
+def obs_preprocess(hx_members, qc):
+    """letkf_obs_read's obs_hx_mean, perturbations (src/letkf_obs.F90:459-466)
+    and QC compaction (:479-489) in FP32: hx_members (k, n).  Returns the
+    indices of the kept obs (qc <= 0, input order), hx (k, nkept) and
+    hx_mean (nkept)."""
+    hx_members = np.asarray(hx_members, np.float32)
+    k = hx_members.shape[0]
+    acc = np.zeros(hx_members.shape[1], np.float32)
+    for m in range(k):
+        acc = acc + hx_members[m]
+    mu = acc / np.float32(k)
+    pert = hx_members - mu[None, :]
+    keep = np.nonzero(np.asarray(qc) <= 0)[0]
+    return keep, np.asfortranarray(pert[:, keep]), np.ascontiguousarray(mu[keep])

@@ -348,3 +348,35 @@ def test_ocean_groups_split():
     assert out["lg_m"].shape[0] == 2
     ij_sat = int(np.argmax(out["lg_m"][0] - out["lg_m"][1]))
     assert out["lg_m"][0, ij_sat] > out["lg_m"][1, ij_sat]
+
+
+def test_state_from_members_closed_forms():
+    """src/letkf_state.F90:326-334 restated in FP32: integer-valued members
+    give exact sums, so mean, perturbations and spread have closed forms; the
+    layout is file order in, state_ij(k, ns, ij) out."""
+    g = np.random.default_rng(21)
+    nij, ns, k = 7, 3, 5
+    mem = g.integers(-20, 20, size=(nij, ns, k)).astype(np.float32)
+    st, mu, sp = O.state_from_members(mem)
+    assert st.shape == (k, ns, nij) and mu.shape == (ns, nij) and sp.shape == (ns, nij)
+    ref_mu = (mem.sum(axis=2) / k).astype(np.float32)
+    assert np.array_equal(mu, ref_mu.T)
+    pert = mem - ref_mu[:, :, None]
+    assert np.array_equal(st, pert.transpose(2, 1, 0))
+    ref_sp = np.sqrt((pert.astype(np.float64) ** 2).sum(axis=2) / (k - 1))
+    assert np.allclose(sp, ref_sp.T, rtol=2e-7)
+    # FP32 order: member by member, not pairwise
+    big = np.full((1, 1, 3), [1e8, 1.0, -1e8], np.float32).reshape(1, 1, 3)
+    assert O.state_from_members(big)[1][0, 0] == np.float32(0.0)
+
+
+def test_obs_preprocess_qc_and_order():
+    """src/letkf_obs.F90:459-489: the mean over members, perturbations,
+    qc > 0 dropped, qc <= 0 (also negative) kept, input order kept."""
+    hx = np.array([[1, 2, 3, 4], [3, 2, 5, 8]], np.float32)     # (k=2, n=4)
+    qc = np.array([0, 1, -3, 0])
+    keep, pert, mu = O.obs_preprocess(hx, qc)
+    assert keep.tolist() == [0, 2, 3]
+    assert mu.tolist() == [2.0, 4.0, 6.0]
+    assert pert.tolist() == [[-1.0, -1.0, -2.0], [1.0, 1.0, 2.0]]
+    assert O.obs_preprocess(hx, np.ones(4))[0].size == 0

@@ -231,6 +231,46 @@ int letkf_gpu_destroy(letkf_gpu_t* h) {
   return LETKF_OK;
 }
 
+int letkf_gpu_set_state_members(letkf_gpu_t* h, const float* members, const float* lat_ij,
+                                const float* lon_ij, const int32_t* mask_ij) {
+  if (!h) return LETKF_EINVAL;
+  const int64_t nij = h->nij, k = h->k, ns = h->ns;
+  if (nij > 0 && (!members || !lat_ij || !lon_ij || !mask_ij))
+    return fail(h, LETKF_EINVAL, "null state array");
+  if (k < 2) return fail(h, LETKF_EINVAL, "ens_size below 2: no spread");
+  HIPCHK(h, dalloc(&h->d_state, (size_t)(k * ns * nij)));
+  HIPCHK(h, dalloc(&h->d_mean, (size_t)(ns * nij)));
+  HIPCHK(h, dalloc(&h->d_sprd, (size_t)(ns * nij)));
+  HIPCHK(h, dalloc(&h->d_lat, (size_t)nij));
+  HIPCHK(h, dalloc(&h->d_lon, (size_t)nij));
+  HIPCHK(h, dalloc(&h->d_mask, (size_t)nij));
+  HIPCHK(h, dalloc(&h->d_gx, (size_t)nij));
+  HIPCHK(h, dalloc(&h->d_gy, (size_t)nij));
+  HIPCHK(h, dalloc(&h->d_gz, (size_t)nij));
+  if (nij > 0) {
+    float* raw = nullptr;
+    HIPCHK(h, dalloc(&raw, (size_t)(k * ns * nij)));
+    hipError_t e = hipMemcpyAsync(raw, members, sizeof(float) * k * ns * nij,
+                                  hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+      launch_state_from_members(h->stream, nij, ns, (int)k, raw, h->d_state, h->d_mean,
+                                h->d_sprd);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    dfree(raw);
+    HIPCHK(h, e);
+    HIPCHK(h, hipMemcpyAsync(h->d_lat, lat_ij, sizeof(float) * nij, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_lon, lon_ij, sizeof(float) * nij, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_mask, mask_ij, sizeof(int32_t) * nij, hipMemcpyHostToDevice, h->stream));
+    launch_grid_xyz(h->stream, nij, h->d_lat, h->d_lon, h->d_gx, h->d_gy, h->d_gz);
+    HIPCHK(h, hipGetLastError());
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->have_state = true;
+  return LETKF_OK;
+}
+
 int letkf_gpu_set_state(letkf_gpu_t* h, const float* state_ij, const float* mean_ij,
                         const float* sprd_ij, const float* lat_ij, const float* lon_ij,
                         const int32_t* mask_ij) {
@@ -264,29 +304,41 @@ int letkf_gpu_set_state(letkf_gpu_t* h, const float* state_ij, const float* mean
 
 static int upload_flags(letkf_gpu_t* h);
 
-int letkf_gpu_set_obs(letkf_gpu_t* h, int64_t n, const float* lat, const float* lon,
-                      const float* zdim, const float* time, const float* val,
-                      const float* err, const int32_t* obsid, const int32_t* platid,
-                      const float* hx_pert, const float* hx_mean) {
+// n input observations from host arrays.  With qc and hx_full (the members'
+// hx, (k, n)) the mean / perturbation split and the QC compaction of
+// letkf_obs_read run on the device first; otherwise hx_pert and hx_mean are
+// the reference's obs_hx and obs_hx_mean after letkf_obs_read.
+static int set_obs_impl(letkf_gpu_t* h, int64_t n, const float* lat, const float* lon,
+                        const float* zdim, const float* time, const float* val,
+                        const float* err, const int32_t* obsid, const int32_t* platid,
+                        const float* hx_pert, const float* hx_mean, const int32_t* qc,
+                        const float* hx_full) {
   if (!h) return LETKF_EINVAL;
   if (n < 0) return fail(h, LETKF_EINVAL, "negative n_obs");
   if (n > 0x7ffffff0) return fail(h, LETKF_EUNSUPPORTED, "n_obs above 2^31");
+  const bool members = hx_full != nullptr;
   if (n > 0 && (!lat || !lon || !zdim || !time || !val || !err || !obsid || !platid ||
-                !hx_pert || !hx_mean))
+                (members ? !qc : (!hx_pert || !hx_mean))))
     return fail(h, LETKF_EINVAL, "null observation array");
   const int64_t k = h->k;
-  h->nobs = n;
-  // raw uploads (freed at the end of the call)
+  // raw uploads and work arrays (freed at the end of the call)
   float *r_lat = nullptr, *r_lon = nullptr, *r_zdim = nullptr, *r_time = nullptr;
   float *r_val = nullptr, *r_err = nullptr, *r_hx = nullptr, *r_hxm = nullptr;
   int32_t *r_obsid = nullptr, *r_platid = nullptr, *cell = nullptr, *scell = nullptr;
-  int32_t *iota = nullptr, *perm = nullptr;
+  int32_t *iota = nullptr, *perm = nullptr, *r_qc = nullptr, *sel = nullptr, *nsel_d = nullptr;
+  float* r_full = nullptr;
+  uint8_t* keep = nullptr;
+  ObsFields cf{};
   void* tmp = nullptr;
   int rc = LETKF_OK;
   auto cleanup = [&]() {
     dfree(r_lat); dfree(r_lon); dfree(r_zdim); dfree(r_time); dfree(r_val); dfree(r_err);
     dfree(r_hx); dfree(r_hxm); dfree(r_obsid); dfree(r_platid); dfree(cell); dfree(scell);
-    dfree(iota); dfree(perm);
+    dfree(iota); dfree(perm); dfree(r_qc); dfree(sel); dfree(nsel_d); dfree(r_full);
+    dfree(keep);
+    for (auto& f : cf.f) dfree(f);
+    for (auto& f : cf.i) dfree(f);
+    dfree(cf.hx);
     if (tmp) hipFree(tmp);
   };
 #define OBSCHK(call)                                                           \
@@ -299,19 +351,12 @@ int letkf_gpu_set_obs(letkf_gpu_t* h, int64_t n, const float* lat, const float* 
       return rc;                                                               \
     }                                                                          \
   } while (0)
-  const size_t nn = (size_t)(n > 0 ? n : 1);
-  OBSCHK(dalloc(&h->d_ox, nn)); OBSCHK(dalloc(&h->d_oy, nn)); OBSCHK(dalloc(&h->d_oz, nn));
-  OBSCHK(dalloc(&h->d_ozdim, nn)); OBSCHK(dalloc(&h->d_oerr, nn)); OBSCHK(dalloc(&h->d_odep, nn));
-  OBSCHK(dalloc(&h->d_otime, nn)); OBSCHK(dalloc(&h->d_oidx, nn)); OBSCHK(dalloc(&h->d_oobsid, nn));
-  OBSCHK(dalloc(&h->d_oplatid, nn)); OBSCHK(dalloc(&h->d_ohx, nn * k));
-  OBSCHK(dalloc(&h->d_cell_start, (size_t)kCells + 1));
-  OBSCHK(dalloc(&h->d_osurf, nn)); OBSCHK(dalloc(&h->d_otwp, nn)); OBSCHK(dalloc(&h->d_otws, nn));
+  hipStream_t st = h->stream;
+  size_t nn = (size_t)(n > 0 ? n : 1);
   OBSCHK(dalloc(&r_lat, nn)); OBSCHK(dalloc(&r_lon, nn)); OBSCHK(dalloc(&r_zdim, nn));
   OBSCHK(dalloc(&r_time, nn)); OBSCHK(dalloc(&r_val, nn)); OBSCHK(dalloc(&r_err, nn));
   OBSCHK(dalloc(&r_hx, nn * k)); OBSCHK(dalloc(&r_hxm, nn)); OBSCHK(dalloc(&r_obsid, nn));
-  OBSCHK(dalloc(&r_platid, nn)); OBSCHK(dalloc(&cell, nn)); OBSCHK(dalloc(&scell, nn));
-  OBSCHK(dalloc(&iota, nn)); OBSCHK(dalloc(&perm, nn));
-  hipStream_t st = h->stream;
+  OBSCHK(dalloc(&r_platid, nn));
   if (n > 0) {
     OBSCHK(hipMemcpyAsync(r_lat, lat, 4 * n, hipMemcpyHostToDevice, st));
     OBSCHK(hipMemcpyAsync(r_lon, lon, 4 * n, hipMemcpyHostToDevice, st));
@@ -319,10 +364,57 @@ int letkf_gpu_set_obs(letkf_gpu_t* h, int64_t n, const float* lat, const float* 
     OBSCHK(hipMemcpyAsync(r_time, time, 4 * n, hipMemcpyHostToDevice, st));
     OBSCHK(hipMemcpyAsync(r_val, val, 4 * n, hipMemcpyHostToDevice, st));
     OBSCHK(hipMemcpyAsync(r_err, err, 4 * n, hipMemcpyHostToDevice, st));
-    OBSCHK(hipMemcpyAsync(r_hx, hx_pert, 4 * n * k, hipMemcpyHostToDevice, st));
-    OBSCHK(hipMemcpyAsync(r_hxm, hx_mean, 4 * n, hipMemcpyHostToDevice, st));
     OBSCHK(hipMemcpyAsync(r_obsid, obsid, 4 * n, hipMemcpyHostToDevice, st));
     OBSCHK(hipMemcpyAsync(r_platid, platid, 4 * n, hipMemcpyHostToDevice, st));
+    if (!members) {
+      OBSCHK(hipMemcpyAsync(r_hx, hx_pert, 4 * n * k, hipMemcpyHostToDevice, st));
+      OBSCHK(hipMemcpyAsync(r_hxm, hx_mean, 4 * n, hipMemcpyHostToDevice, st));
+    }
+  }
+  if (members && n > 0) {
+    // obs_hx_mean, perturbations and the qc <= 0 flags (src/letkf_obs.F90:459-466)
+    OBSCHK(dalloc(&r_full, nn * k)); OBSCHK(dalloc(&r_qc, nn)); OBSCHK(dalloc(&keep, nn));
+    OBSCHK(dalloc(&sel, nn)); OBSCHK(dalloc(&nsel_d, 1));
+    OBSCHK(hipMemcpyAsync(r_full, hx_full, 4 * n * k, hipMemcpyHostToDevice, st));
+    OBSCHK(hipMemcpyAsync(r_qc, qc, 4 * n, hipMemcpyHostToDevice, st));
+    launch_obs_hx_split(st, n, (int)k, r_full, r_qc, r_hx, r_hxm, keep);
+    OBSCHK(hipGetLastError());
+    // qc > 0 dropped, order kept (:479-489)
+    hipcub::CountingInputIterator<int32_t> idx(0);
+    size_t tb = 0;
+    OBSCHK(hipcub::DeviceSelect::Flagged(nullptr, tb, idx, keep, sel, nsel_d, (int)n, st));
+    OBSCHK(hipMalloc(&tmp, tb));
+    OBSCHK(hipcub::DeviceSelect::Flagged(tmp, tb, idx, keep, sel, nsel_d, (int)n, st));
+    int32_t nsel = 0;
+    OBSCHK(hipMemcpyAsync(&nsel, nsel_d, 4, hipMemcpyDeviceToHost, st));
+    OBSCHK(hipStreamSynchronize(st));
+    hipFree(tmp);
+    tmp = nullptr;
+    const size_t ns1 = (size_t)(nsel > 0 ? nsel : 1);
+    for (auto& f : cf.f) OBSCHK(dalloc(&f, ns1));
+    for (auto& f : cf.i) OBSCHK(dalloc(&f, ns1));
+    OBSCHK(dalloc(&cf.hx, ns1 * k));
+    const ObsFields in{{r_lat, r_lon, r_zdim, r_time, r_val, r_err, r_hxm},
+                       {r_obsid, r_platid}, r_hx};
+    launch_obs_compact(st, nsel, (int)k, sel, in, cf);
+    OBSCHK(hipGetLastError());
+    std::swap(r_lat, cf.f[0]); std::swap(r_lon, cf.f[1]); std::swap(r_zdim, cf.f[2]);
+    std::swap(r_time, cf.f[3]); std::swap(r_val, cf.f[4]); std::swap(r_err, cf.f[5]);
+    std::swap(r_hxm, cf.f[6]); std::swap(r_obsid, cf.i[0]); std::swap(r_platid, cf.i[1]);
+    std::swap(r_hx, cf.hx);
+    n = nsel;
+    nn = ns1;
+  }
+  h->nobs = n;
+  OBSCHK(dalloc(&h->d_ox, nn)); OBSCHK(dalloc(&h->d_oy, nn)); OBSCHK(dalloc(&h->d_oz, nn));
+  OBSCHK(dalloc(&h->d_ozdim, nn)); OBSCHK(dalloc(&h->d_oerr, nn)); OBSCHK(dalloc(&h->d_odep, nn));
+  OBSCHK(dalloc(&h->d_otime, nn)); OBSCHK(dalloc(&h->d_oidx, nn)); OBSCHK(dalloc(&h->d_oobsid, nn));
+  OBSCHK(dalloc(&h->d_oplatid, nn)); OBSCHK(dalloc(&h->d_ohx, nn * k));
+  OBSCHK(dalloc(&h->d_cell_start, (size_t)kCells + 1));
+  OBSCHK(dalloc(&h->d_osurf, nn)); OBSCHK(dalloc(&h->d_otwp, nn)); OBSCHK(dalloc(&h->d_otws, nn));
+  OBSCHK(dalloc(&cell, nn)); OBSCHK(dalloc(&scell, nn));
+  OBSCHK(dalloc(&iota, nn)); OBSCHK(dalloc(&perm, nn));
+  if (n > 0) {
     launch_obs_prep(st, n, (int)k, r_lat, r_lon, cell, iota);
     OBSCHK(hipGetLastError());
     size_t tb = 0;
@@ -345,6 +437,24 @@ int letkf_gpu_set_obs(letkf_gpu_t* h, int64_t n, const float* lat, const float* 
   h->have_obs = true;
   if (h->have_loc) return upload_flags(h);
   return LETKF_OK;
+}
+
+int letkf_gpu_set_obs(letkf_gpu_t* h, int64_t n, const float* lat, const float* lon,
+                      const float* zdim, const float* time, const float* val,
+                      const float* err, const int32_t* obsid, const int32_t* platid,
+                      const float* hx_pert, const float* hx_mean) {
+  return set_obs_impl(h, n, lat, lon, zdim, time, val, err, obsid, platid, hx_pert, hx_mean,
+                      nullptr, nullptr);
+}
+
+int letkf_gpu_set_obs_members(letkf_gpu_t* h, int64_t n, const float* lat, const float* lon,
+                              const float* zdim, const float* time, const float* val,
+                              const float* err, const int32_t* obsid, const int32_t* platid,
+                              const int32_t* qc, const float* hx_members) {
+  if (n > 0 && !hx_members) return h ? fail(h, LETKF_EINVAL, "null hx_members") : LETKF_EINVAL;
+  static const float kNone = 0.0f;
+  return set_obs_impl(h, n, lat, lon, zdim, time, val, err, obsid, platid, nullptr, nullptr, qc,
+                      hx_members ? hx_members : &kNone);
 }
 
 static void build_interp(const letkf_linearinterp_lat& c, int* n, float* lat, float* dist) {
@@ -795,6 +905,36 @@ int letkf_gpu_get_state_f64(letkf_gpu_t* h, double* state, double* mean, double*
   if (state) HIPCHK(h, hipMemcpy(state, h->d_ana, 8 * k * ns * nij, hipMemcpyDeviceToHost));
   if (mean) HIPCHK(h, hipMemcpy(mean, h->d_amean, 8 * ns * nij, hipMemcpyDeviceToHost));
   if (sprd) HIPCHK(h, hipMemcpy(sprd, h->d_asprd, 8 * ns * nij, hipMemcpyDeviceToHost));
+  return LETKF_OK;
+}
+
+int letkf_gpu_get_state_members(letkf_gpu_t* h, float* members, float* mean, float* sprd) {
+  if (!h) return LETKF_EINVAL;
+  if (!h->have_run) return fail(h, LETKF_ESTATE, "no analysis: call letkf_gpu_run first");
+  const int64_t k = h->k, ns = h->ns, nij = h->nij;
+  if (nij * ns <= 0) return LETKF_OK;
+  float *dm = nullptr, *dmu = nullptr, *dsp = nullptr;
+  auto done = [&](hipError_t e) {
+    dfree(dm); dfree(dmu); dfree(dsp);
+    return e;
+  };
+  hipError_t e = hipSuccess;
+  if (members && e == hipSuccess) e = dalloc(&dm, (size_t)(k * ns * nij));
+  if (mean && e == hipSuccess) e = dalloc(&dmu, (size_t)(ns * nij));
+  if (sprd && e == hipSuccess) e = dalloc(&dsp, (size_t)(ns * nij));
+  if (e == hipSuccess) {
+    launch_members_from_state(h->stream, nij, ns, (int)k, h->d_ana, h->d_amean, h->d_asprd, dm,
+                              dmu, dsp);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && members)
+    e = hipMemcpyAsync(members, dm, 4 * k * ns * nij, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess && mean)
+    e = hipMemcpyAsync(mean, dmu, 4 * ns * nij, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess && sprd)
+    e = hipMemcpyAsync(sprd, dsp, 4 * ns * nij, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  HIPCHK(h, done(e));
   return LETKF_OK;
 }
 

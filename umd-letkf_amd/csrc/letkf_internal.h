@@ -153,6 +153,23 @@ void launch_obs_flags(hipStream_t st, const LocDev& L, int64_t n,
                       float* tw_prof, float* tw_sat);
 void launch_obs_pad_hx(hipStream_t st, int64_t n, int k, int kp, const float* ohx,
                        const float* odep, float* hxp);
+// observation arrays of set_obs in input order: lat, lon, zdim, time, val,
+// err, hx_mean (f); obsid, platid (i); hx perturbations (k, n)
+struct ObsFields {
+  static constexpr int kF = 7, kI = 2;
+  float* f[kF];
+  int32_t* i[kI];
+  float* hx;
+};
+void launch_state_from_members(hipStream_t st, int64_t nij, int64_t ns, int k,
+                               const float* members, float* state, float* mean, float* sprd);
+void launch_members_from_state(hipStream_t st, int64_t nij, int64_t ns, int k,
+                               const double* ana, const double* amean, const double* asprd,
+                               float* members, float* mean, float* sprd);
+void launch_obs_hx_split(hipStream_t st, int64_t n, int k, const float* hx, const int32_t* qc,
+                         float* pert, float* hmean, uint8_t* keep);
+void launch_obs_compact(hipStream_t st, int64_t nsel, int k, const int32_t* sel,
+                        const ObsFields& in, const ObsFields& out);
 void launch_grid_xyz(hipStream_t st, int64_t n, const float* lat,
                      const float* lon, double* x, double* y, double* z);
 void launch_search(hipStream_t st, bool fill, const LocDev& L,

@@ -83,7 +83,8 @@ MODULE letkf_gpu_mod
        letkf_gpu_set_state, letkf_gpu_set_obs, letkf_gpu_set_statevars, &
        letkf_gpu_set_localizer, letkf_gpu_set_solver, letkf_gpu_run, letkf_gpu_get_state, &
        letkf_gpu_get_state_f64, letkf_gpu_get_diag, letkf_gpu_maxgroups, &
-       letkf_gpu_get_selection, letkf_gpu_get_timing, letkf_gpu_errmsg
+       letkf_gpu_get_selection, letkf_gpu_get_timing, letkf_gpu_errmsg, &
+       letkf_gpu_set_state_members, letkf_gpu_set_obs_members, letkf_gpu_get_state_members
 
   INTERFACE
      FUNCTION letkf_gpu_version() BIND(C, name="letkf_gpu_version")
@@ -209,6 +210,34 @@ MODULE letkf_gpu_mod
        TYPE(letkf_gpu_timing), INTENT(out) :: t
        INTEGER(c_int) :: letkf_gpu_get_timing
      END FUNCTION letkf_gpu_get_timing
+
+     FUNCTION letkf_gpu_set_state_members(h, members, lat_ij, lon_ij, mask_ij) &
+          BIND(C, name="letkf_gpu_set_state_members")
+       IMPORT :: c_int, c_ptr, c_float, c_int32_t
+       TYPE(c_ptr), VALUE :: h
+       REAL(c_float), INTENT(in) :: members(*), lat_ij(*), lon_ij(*)
+       INTEGER(c_int32_t), INTENT(in) :: mask_ij(*)
+       INTEGER(c_int) :: letkf_gpu_set_state_members
+     END FUNCTION letkf_gpu_set_state_members
+
+     FUNCTION letkf_gpu_set_obs_members(h, n_obs, lat, lon, zdim, time, val, err, obsid, &
+          platid, qc, hx_members) BIND(C, name="letkf_gpu_set_obs_members")
+       IMPORT :: c_int, c_ptr, c_float, c_int32_t, c_int64_t
+       TYPE(c_ptr), VALUE :: h
+       INTEGER(c_int64_t), VALUE :: n_obs
+       REAL(c_float), INTENT(in) :: lat(*), lon(*), zdim(*), time(*), val(*), err(*)
+       INTEGER(c_int32_t), INTENT(in) :: obsid(*), platid(*), qc(*)
+       REAL(c_float), INTENT(in) :: hx_members(*)
+       INTEGER(c_int) :: letkf_gpu_set_obs_members
+     END FUNCTION letkf_gpu_set_obs_members
+
+     FUNCTION letkf_gpu_get_state_members(h, members, mean_ij, sprd_ij) &
+          BIND(C, name="letkf_gpu_get_state_members")
+       IMPORT :: c_int, c_ptr, c_float
+       TYPE(c_ptr), VALUE :: h
+       REAL(c_float), INTENT(out) :: members(*), mean_ij(*), sprd_ij(*)
+       INTEGER(c_int) :: letkf_gpu_get_state_members
+     END FUNCTION letkf_gpu_get_state_members
 
      FUNCTION c_strlen(s) BIND(C, name="strlen")
        IMPORT :: c_ptr, c_size_t

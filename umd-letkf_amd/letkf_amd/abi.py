@@ -87,6 +87,8 @@ EXPORTED = [
     "letkf_gpu_set_solver", "letkf_gpu_run", "letkf_gpu_get_state",
     "letkf_gpu_get_state_f64", "letkf_gpu_get_diag", "letkf_gpu_maxgroups",
     "letkf_gpu_get_selection", "letkf_gpu_get_timing",
+    "letkf_gpu_set_state_members", "letkf_gpu_set_obs_members",
+    "letkf_gpu_get_state_members",
 ]
 
 _FP = C.POINTER(C.c_float)
@@ -113,6 +115,10 @@ _SIGS = {
     "letkf_gpu_get_selection": (C.c_int, [_H, C.c_int64, C.c_int32, _IP, _FP,
                                           C.c_int64, C.POINTER(C.c_int64)]),
     "letkf_gpu_get_timing": (C.c_int, [_H, C.POINTER(Timing)]),
+    "letkf_gpu_set_state_members": (C.c_int, [_H, _FP, _FP, _FP, _IP]),
+    "letkf_gpu_set_obs_members": (C.c_int, [_H, C.c_int64, _FP, _FP, _FP, _FP, _FP,
+                                            _FP, _IP, _IP, _IP, _FP]),
+    "letkf_gpu_get_state_members": (C.c_int, [_H, _FP, _FP, _FP]),
 }
 
 _lib = None

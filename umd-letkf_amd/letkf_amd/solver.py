@@ -25,7 +25,14 @@ class LetkfError(RuntimeError):
 
 class LetkfGpuSolver:
     def __init__(self, prob: Problem, precision: str = "fp64", path: str = "plain",
-                 max_obs: int = 0):
+                 max_obs: int = 0, members=None, obs_members=None):
+        """members: optional (nij, ns, k) full member values in file order,
+        uploaded with letkf_gpu_set_state_members instead of prob's
+        perturbations, mean and spread.  obs_members: optional (qc,
+        hx_members (k, nobs)) uploaded with letkf_gpu_set_obs_members
+        instead of prob's hx and hx mean."""
+        self._members = members
+        self._obs_members = obs_members
         self.lib = abi.load_library()
         self.prob = prob
         self.h = C.c_void_p()
@@ -51,14 +58,31 @@ class LetkfGpuSolver:
         p = self.prob
         p.validate()
         F, I = C.c_float, C.c_int32
-        self._chk(self.lib.letkf_gpu_set_state(
-            self.h, abi.ptr(p.state, F), abi.ptr(p.mean, F), abi.ptr(p.sprd, F),
-            abi.ptr(p.lat, F), abi.ptr(p.lon, F), abi.ptr(p.mask, I)))
-        self._chk(self.lib.letkf_gpu_set_obs(
-            self.h, p.nobs, abi.ptr(p.olat, F), abi.ptr(p.olon, F), abi.ptr(p.ozdim, F),
-            abi.ptr(p.otime, F), abi.ptr(p.oval, F), abi.ptr(p.oerr, F),
-            abi.ptr(p.obsid, I), abi.ptr(p.platid, I), abi.ptr(p.hx, F),
-            abi.ptr(p.hxmean, F)))
+        if self._members is None:
+            self._chk(self.lib.letkf_gpu_set_state(
+                self.h, abi.ptr(p.state, F), abi.ptr(p.mean, F), abi.ptr(p.sprd, F),
+                abi.ptr(p.lat, F), abi.ptr(p.lon, F), abi.ptr(p.mask, I)))
+        else:
+            mem = np.asfortranarray(self._members, np.float32)
+            assert mem.shape == (p.nij, p.ns, p.k)
+            self._chk(self.lib.letkf_gpu_set_state_members(
+                self.h, abi.ptr(mem, F), abi.ptr(p.lat, F), abi.ptr(p.lon, F),
+                abi.ptr(p.mask, I)))
+        if self._obs_members is None:
+            self._chk(self.lib.letkf_gpu_set_obs(
+                self.h, p.nobs, abi.ptr(p.olat, F), abi.ptr(p.olon, F), abi.ptr(p.ozdim, F),
+                abi.ptr(p.otime, F), abi.ptr(p.oval, F), abi.ptr(p.oerr, F),
+                abi.ptr(p.obsid, I), abi.ptr(p.platid, I), abi.ptr(p.hx, F),
+                abi.ptr(p.hxmean, F)))
+        else:
+            qc, hxm = self._obs_members
+            qc = np.ascontiguousarray(qc, np.int32)
+            hxm = np.asfortranarray(hxm, np.float32)
+            assert qc.shape == (p.nobs,) and hxm.shape == (p.k, p.nobs)
+            self._chk(self.lib.letkf_gpu_set_obs_members(
+                self.h, p.nobs, abi.ptr(p.olat, F), abi.ptr(p.olon, F), abi.ptr(p.ozdim, F),
+                abi.ptr(p.otime, F), abi.ptr(p.oval, F), abi.ptr(p.oerr, F),
+                abi.ptr(p.obsid, I), abi.ptr(p.platid, I), abi.ptr(qc, I), abi.ptr(hxm, F)))
         bounds = np.ascontiguousarray(p.bounds.T, np.float32)
         self._chk(self.lib.letkf_gpu_set_statevars(
             self.h, p.nvars, abi.ptr(p.var_s, I), abi.ptr(p.var_lev, I),
@@ -86,6 +110,18 @@ class LetkfGpuSolver:
         self._chk(self.lib.letkf_gpu_get_state_f64(self.h, abi.ptr(st, D), abi.ptr(mn, D),
                                                    abi.ptr(sp, D)))
         return st, mn, sp
+
+    def state_members(self):
+        """Analysis in file order: members (nij, ns, k), mean and spread
+        (nij, ns), FP32 (letkf_gpu_get_state_members)."""
+        p = self.prob
+        mem = np.zeros((p.nij, p.ns, p.k), np.float32, order="F")
+        mu = np.zeros((p.nij, p.ns), np.float32, order="F")
+        sp = np.zeros((p.nij, p.ns), np.float32, order="F")
+        F = C.c_float
+        self._chk(self.lib.letkf_gpu_get_state_members(self.h, abi.ptr(mem, F),
+                                                       abi.ptr(mu, F), abi.ptr(sp, F)))
+        return mem, mu, sp
 
     def state_f32(self):
         p = self.prob
