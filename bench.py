@@ -12,8 +12,12 @@ stream, with the HBM traffic recorded from rocprofv3 counters in
 profiles/traffic.json), the CPU baseline (the FP64 oracle on a bounded
 sample of gridpoints, all host threads up to 16) and the error of the GPU
 analysis against that oracle on the same sample.
+
+--config c5 runs the cycled analysis instead (ten cycles on the same grid,
+fresh observations each cycle) and prints per-cycle latency percentiles.
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -155,6 +159,110 @@ def cpu_baseline(prob, solver, seconds, nthreads):
     return base, err
 
 
+def bench_cycles(args):
+    """Cycled analysis on the 0.25-degree grid (k=50): ``args.cycles``
+    cycles, each background the previous batched analysis (file order)
+    moved one gridpoint east plus N(0, 0.1^2), each with fresh swath obs.
+    The batched and the plain FP64 handle each take the same cycle inputs
+    through set_state_members / set_obs_members, run, and return the
+    analysis through get_state_members; the batched analysis drives the
+    next cycle.  Per cycle: wall latency of the run and of the whole cycle
+    (uploads, device preprocessing, run, egress), and the FP64 analysis of
+    both paths against the oracle on a random sample of gridpoints."""
+    import oracle as O
+    from letkf_amd import synth
+    from letkf_amd.solver import LetkfGpuSolver
+
+    nt = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.time()
+    prob = synth.config3()
+    mem = np.asfortranarray((prob.state + prob.mean[None]).transpose(2, 1, 0), np.float32)
+    log(f"[cycles] cycle 0 inputs ({time.time() - t0:.1f} s)")
+    truth = None
+    paths = (("batched", "batched", args.precision), ("plain", "plain", "fp64"))
+    solvers = {}
+    rows = []
+    g = np.random.default_rng(4321)
+    pool = np.nonzero(prob.mask == 0)[0]
+
+    def rel(a, b):
+        den = np.abs(b).max()
+        return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
+
+    for c in range(args.cycles):
+        if c > 0:
+            t0 = time.time()
+            prob = synth.config5_cycle(prob, mem, c)
+            log(f"[cycles] cycle {c} inputs: {prob.nobs} obs ({time.time() - t0:.1f} s)")
+        truth = prob.meta["truth"].ravel()
+        qc = np.zeros(prob.nobs, np.int32)
+        hxm = np.asfortranarray((prob.hx + prob.hxmean[None]).astype(np.float32))
+        # the oracle's inputs as the device's ingest makes them (FP32, member order)
+        ost, omn, osp = O.state_from_members(mem)
+        _, ohx, ohm = O.obs_preprocess(hxm, qc)
+        oprob = dataclasses.replace(prob, state=ost, mean=omn, sprd=osp, hx=ohx, hxmean=ohm)
+        pts = np.sort(g.choice(pool, min(args.cycle_sample, pool.size), replace=False))
+        orc = O.analysis(oprob, points=pts, nthreads=nt)
+        bg = omn[0]
+        rec = {"cycle": c, "n_obs": int(prob.nobs),
+               "bg_rmse": float(np.sqrt(np.mean((bg[pool] - truth[pool]) ** 2)))}
+        for name, path, prec in paths:
+            s = solvers.get(name)
+            if s is None:
+                # handle creation and one untimed warm-up run on cycle 0
+                s = LetkfGpuSolver(prob, precision=prec, path=path, members=mem,
+                                   obs_members=(qc, hxm))
+                s.run()
+                solvers[name] = s
+            ta = time.perf_counter()
+            s.cycle(prob, members=mem, obs_members=(qc, hxm))
+            tb = time.perf_counter()
+            s.run()
+            tc = time.perf_counter()
+            amem, amn, asp = s.state_members()
+            td = time.perf_counter()
+            st, mn, sp = s.state_f64()
+            d = s.diag()
+            sel_ok = bool(np.array_equal(d["lg_selhash"][pts, :].T, orc["lg_hash"]))
+            rec[name] = {"input_ms": round(1e3 * (tb - ta), 2), "run_ms": round(1e3 * (tc - tb), 2),
+                         "egress_ms": round(1e3 * (td - tc), 2),
+                         "cycle_ms": round(1e3 * (td - ta), 2),
+                         "mean_rel": rel(mn[:, pts], orc["mean"]),
+                         "sprd_rel": rel(sp[:, pts], orc["sprd"]),
+                         "selection_bit_exact": sel_ok,
+                         "ana_rmse": float(np.sqrt(np.mean((amn[pool, 0] - truth[pool]) ** 2)))}
+            if name == "batched":
+                nxt = amem
+        log(f"[cycles] {json.dumps(rec)}")
+        rows.append(rec)
+        mem = nxt
+    for s in solvers.values():
+        s.final()
+
+    def pct(name, key):
+        v = np.array([r[name][key] for r in rows])
+        return {"p50": round(float(np.percentile(v, 50)), 2),
+                "p99": round(float(np.percentile(v, 99)), 2)}
+    summ = {nm: {kk: pct(nm, kk) for kk in ("run_ms", "cycle_ms", "input_ms", "egress_ms")}
+            for nm, _, _ in paths}
+    worst = {nm: {"mean_rel": max(r[nm]["mean_rel"] for r in rows),
+                  "sprd_rel": max(r[nm]["sprd_rel"] for r in rows),
+                  "selection_bit_exact": all(r[nm]["selection_bit_exact"] for r in rows)}
+             for nm, _, _ in paths}
+    out = {"metric": "per-cycle analysis latency, cycled 0.25deg k=50 (p50/p99 over cycles)",
+           "value": summ["batched"]["run_ms"]["p50"], "unit": "ms", "n_gpus": 1,
+           "higher_is_better": False, "cycles": args.cycles,
+           "dtype": args.precision, "data": "synthetic (seeded Philox)",
+           "config": {"workload": "C5: 10-cycle 0.25deg 1440x720 surface, k=50, 1-2M fresh "
+                                  "swath obs per cycle, background = analysis shifted 1 gp "
+                                  "east + N(0, 0.1^2)", "n_gp": prob.nij, "k": prob.k,
+                      "precision": args.precision, "parallelism": "single GPU"},
+           "latency_ms": summ, "error_vs_fp64_oracle_worst": worst,
+           "oracle_sample_per_cycle": int(min(args.cycle_sample, pool.size)),
+           "cycles_detail": rows}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +281,9 @@ def main():
                     help="record the product kernel's per-dispatch HBM traffic from two "
                          "rocprofv3 --pmc counter_collection.csv files in profiles/traffic.json")
     ap.add_argument("--problems-per-dispatch", type=int, default=0)
+    ap.add_argument("--cycles", type=int, default=10, help="c5: number of analysis cycles")
+    ap.add_argument("--cycle-sample", type=int, default=1000,
+                    help="c5: gridpoints per cycle checked against the FP64 oracle")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, args.config, args.path, args.precision,
@@ -182,6 +293,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; "
                           "the analysis shards by gridpoint across GPUs (see DESIGN.md)"}))
         sys.exit(2)
+
+    if args.config == "c5":
+        bench_cycles(args)
+        return
 
     from letkf_amd.solver import LetkfGpuSolver
 

@@ -57,6 +57,20 @@ class LetkfGpuSolver:
     def _upload(self):
         p = self.prob
         p.validate()
+        self._upload_state()
+        self._upload_obs()
+        F, I = C.c_float, C.c_int32
+        bounds = np.ascontiguousarray(p.bounds.T, np.float32)
+        self._chk(self.lib.letkf_gpu_set_statevars(
+            self.h, p.nvars, abi.ptr(p.var_s, I), abi.ptr(p.var_lev, I),
+            abi.ptr(p.inc_max, F), abi.ptr(bounds, F)))
+        desc, keep = p.loc.to_c()
+        self._chk(self.lib.letkf_gpu_set_localizer(self.h, C.byref(desc)))
+        del keep
+        self._chk(self.lib.letkf_gpu_set_solver(self.h, p.rtps, p.rtpp, p.mul))
+
+    def _upload_state(self):
+        p = self.prob
         F, I = C.c_float, C.c_int32
         if self._members is None:
             self._chk(self.lib.letkf_gpu_set_state(
@@ -68,6 +82,10 @@ class LetkfGpuSolver:
             self._chk(self.lib.letkf_gpu_set_state_members(
                 self.h, abi.ptr(mem, F), abi.ptr(p.lat, F), abi.ptr(p.lon, F),
                 abi.ptr(p.mask, I)))
+
+    def _upload_obs(self):
+        p = self.prob
+        F, I = C.c_float, C.c_int32
         if self._obs_members is None:
             self._chk(self.lib.letkf_gpu_set_obs(
                 self.h, p.nobs, abi.ptr(p.olat, F), abi.ptr(p.olon, F), abi.ptr(p.ozdim, F),
@@ -83,16 +101,20 @@ class LetkfGpuSolver:
                 self.h, p.nobs, abi.ptr(p.olat, F), abi.ptr(p.olon, F), abi.ptr(p.ozdim, F),
                 abi.ptr(p.otime, F), abi.ptr(p.oval, F), abi.ptr(p.oerr, F),
                 abi.ptr(p.obsid, I), abi.ptr(p.platid, I), abi.ptr(qc, I), abi.ptr(hxm, F)))
-        bounds = np.ascontiguousarray(p.bounds.T, np.float32)
-        self._chk(self.lib.letkf_gpu_set_statevars(
-            self.h, p.nvars, abi.ptr(p.var_s, I), abi.ptr(p.var_lev, I),
-            abi.ptr(p.inc_max, F), abi.ptr(bounds, F)))
-        desc, keep = p.loc.to_c()
-        self._chk(self.lib.letkf_gpu_set_localizer(self.h, C.byref(desc)))
-        del keep
-        self._chk(self.lib.letkf_gpu_set_solver(self.h, p.rtps, p.rtpp, p.mul))
 
     # ------------------------------------------------------------ API
+    def cycle(self, prob: Problem, members=None, obs_members=None):
+        """The next cycle's inputs on the same handle: a new background and
+        new observations (prob's fields, or members / obs_members as in the
+        constructor).  Grid, state variables, localizer and inflation stay
+        as they were set; prob must have the same k, ns and nij."""
+        assert (prob.k, prob.ns, prob.nij) == (self.prob.k, self.prob.ns, self.prob.nij)
+        self.prob = prob
+        self._members = members
+        self._obs_members = obs_members
+        self._upload_state()
+        self._upload_obs()
+
     def run(self):
         self._chk(self.lib.letkf_gpu_run(self.h))
 

@@ -291,7 +291,38 @@ def config3(seed=0x1E7F0003, nx=1440, ny=720, k=50, nobs=2_000_000,
     return make_problem(members, lat_ij, lon_ij, np.zeros(ny * nx, bool), olat, olon,
                         z, z, val, np.full(nobs, sigma_o ** 2), np.ones(nobs),
                         np.ones(nobs), hx, [0], [1], novrt(radius), rtps=rtps,
-                        meta=dict(config=3, nx=nx, ny=ny, seed=seed))
+                        meta=dict(config=3, nx=nx, ny=ny, seed=seed,
+                                  truth=truth.astype(np.float32)))
+
+
+def config5_cycle(prev, members_fo, cycle, seed=0x1E7F0005, nobs=(1_000_000, 2_000_000),
+                  sigma_noise=0.1, sigma_o=0.25, **swath):
+    """Cycle ``cycle`` (>= 1) of the cycled 0.25-degree analysis (config 5).
+
+    The background is the previous analysis ``members_fo`` (nij, 1, k) in
+    file order, advected by one gridpoint eastward plus N(0, sigma_noise^2)
+    per member and gridpoint; the truth moves one gridpoint with it.  A
+    fresh swath set (its own seed, so a different track phase) of between
+    nobs[0] and nobs[1] observations samples the new truth.  ``prev`` is the
+    previous cycle's Problem (grid, localizer and settings are kept)."""
+    g = rng(seed + 7919 * cycle)
+    nx, ny = prev.meta["nx"], prev.meta["ny"]
+    k = prev.k
+    lon, lat, lon_ij, lat_ij = flat_grid(nx, ny)
+    mem = np.asarray(members_fo, np.float32).reshape(ny, nx, k)
+    mem = np.roll(mem, 1, axis=1) + g.normal(0.0, sigma_noise, mem.shape).astype(np.float32)
+    members = np.ascontiguousarray(mem.transpose(2, 0, 1))           # (k, ny, nx)
+    truth = np.roll(prev.meta["truth"], 1, axis=1)
+    n = int(g.integers(nobs[0], nobs[1] + 1))
+    olat, olon = swath_points(g, n, **swath)
+    hx = bilinear(members, lat, lon, olat, olon)
+    val = bilinear(truth, lat, lon, olat, olon) + g.normal(0.0, sigma_o, n)
+    z = np.zeros(n)
+    return make_problem(members.reshape(k, 1, ny * nx), lat_ij, lon_ij, prev.mask != 0,
+                        olat, olon, z, z, val, np.full(n, sigma_o ** 2), np.ones(n),
+                        np.ones(n), hx, [0], [1], prev.loc, rtps=prev.rtps,
+                        meta=dict(config=5, cycle=cycle, nx=nx, ny=ny, seed=seed,
+                                  truth=truth))
 
 
 # ---------------------------------------------------------------- config 2
