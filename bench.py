@@ -168,7 +168,9 @@ def bench_cycles(args):
     analysis through get_state_members; the batched analysis drives the
     next cycle.  Per cycle: wall latency of the run and of the whole cycle
     (uploads, device preprocessing, run, egress), and the FP64 analysis of
-    both paths against the oracle on a random sample of gridpoints."""
+    both paths against the oracle on a random sample of gridpoints.  Each
+    handle is created, autotuned and warmed up on cycle 0, outside the
+    per-cycle timings."""
     import oracle as O
     from letkf_amd import synth
     from letkf_amd.solver import LetkfGpuSolver
@@ -212,7 +214,11 @@ def bench_cycles(args):
                 # handle creation and one untimed warm-up run on cycle 0
                 s = LetkfGpuSolver(prob, precision=prec, path=path, members=mem,
                                    obs_members=(qc, hxm))
-                s.run()
+                if args.no_autotune:
+                    s.run()
+                else:
+                    # tuned once on cycle 0, kept for every cycle
+                    log(f"[cycles] {name} autotune: {s.autotune(repeats=1)}")
                 solvers[name] = s
             ta = time.perf_counter()
             s.cycle(prob, members=mem, obs_members=(qc, hxm))
@@ -281,6 +287,8 @@ def main():
                     help="record the product kernel's per-dispatch HBM traffic from two "
                          "rocprofv3 --pmc counter_collection.csv files in profiles/traffic.json")
     ap.add_argument("--problems-per-dispatch", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch choices instead of letkf_gpu_autotune")
     ap.add_argument("--cycles", type=int, default=10, help="c5: number of analysis cycles")
     ap.add_argument("--cycle-sample", type=int, default=1000,
                     help="c5: gridpoints per cycle checked against the FP64 oracle")
@@ -308,6 +316,10 @@ def main():
     solver = LetkfGpuSolver(prob, precision=args.precision, path=args.path)
     upload_s = time.perf_counter() - tu
     log(f"[bench] inputs uploaded ({upload_s:.2f} s)")
+    tuning = None
+    if not args.no_autotune:
+        tuning = solver.autotune(repeats=2)
+        log(f"[bench] autotune: {tuning}")
     for i in range(args.warmup):
         solver.run()
         log(f"[bench] warmup {i} done")
@@ -339,6 +351,7 @@ def main():
                    "path": args.path, "precision": args.precision,
                    "parallelism": "single GPU"},
         "roofline": roof,
+        "tuning": tuning,
         "phases_ms": {"search": round(tavg["search_ms"], 3),
                       "order": round(tavg["order_ms"], 3),
                       "product": round(tavg["product_ms"], 3),
@@ -381,6 +394,8 @@ def main():
         log(f"[bench] ingest {out['ingest']}")
     if not args.no_fp64_compare and not (args.path == "plain" and args.precision == "fp64"):
         ref = LetkfGpuSolver(prob, precision="fp64", path="plain")
+        if not args.no_autotune:
+            ref.autotune(repeats=1)
         for _ in range(args.warmup):
             ref.run()
         t0 = time.perf_counter()

@@ -134,6 +134,22 @@ typedef struct letkf_gpu_timing {
                           product_ms above is then the product kernel alone */
 } letkf_gpu_timing;
 
+/* Launch choices of the analysis (no counterpart in the reference, which
+ * has no device kernels to tune).  order: problem order of the batched
+ * product, 0 gridpoint order, 1 longest first, 2 buckets of the kept
+ * count largest first, 3 buckets with a Morton walk inside; bucket_bits:
+ * the bucket edges are the powers of two split into 2^bucket_bits parts
+ * (0..2); tridiag_reg: 1 register-resident Householder reduction for
+ * 32 <= k < 64, 0 the LDS kernel.  None of them changes the analysis. */
+typedef struct letkf_gpu_tuning {
+  int32_t order;
+  int32_t bucket_bits;
+  int32_t tridiag_reg;
+  int32_t n_tried;     /* autotune: candidates timed                  */
+  double best_ms;      /* autotune: analysis time of the chosen one   */
+  double default_ms;   /* autotune: analysis time of the defaults     */
+} letkf_gpu_tuning;
+
 typedef struct letkf_gpu_t letkf_gpu_t;
 
 /* Library version string. */
@@ -249,6 +265,19 @@ int letkf_gpu_get_selection(letkf_gpu_t* h, int64_t ij, int32_t group,
 
 /* Phase timings and work counts of the last run. */
 int letkf_gpu_get_timing(const letkf_gpu_t* h, letkf_gpu_timing* t);
+
+/* Autotuner (no counterpart in the reference): times the analysis of the
+ * handle's current inputs under each launch choice (coordinate search:
+ * order and bucket edges, then the tridiagonalization kernel), `repeats`
+ * runs each (best taken, 1..16), keeps the fastest in the handle and
+ * returns it.  Needs everything letkf_gpu_run needs; leaves the analysis
+ * of the last timed run, identical under every choice. */
+int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out);
+
+/* Read or set the launch choices (for example a tuning saved from an
+ * earlier autotune of the same configuration). */
+int letkf_gpu_get_tuning(const letkf_gpu_t* h, letkf_gpu_tuning* out);
+int letkf_gpu_set_tuning(letkf_gpu_t* h, const letkf_gpu_tuning* t);
 
 #ifdef __cplusplus
 }

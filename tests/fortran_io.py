@@ -73,4 +73,4 @@ def struct_sizes():
 
 def ctypes_sizes():
     return [C.sizeof(x) for x in (abi.Dims, abi.LinearInterpLat, abi.LocDesc, abi.Diag,
-                                  abi.Timing)]
+                                  abi.Timing, abi.Tuning)]

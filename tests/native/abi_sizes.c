@@ -11,6 +11,7 @@ int main(void) {
   printf("letkf_loc_desc %zu\n", sizeof(letkf_loc_desc));
   printf("letkf_gpu_diag %zu\n", sizeof(letkf_gpu_diag));
   printf("letkf_gpu_timing %zu\n", sizeof(letkf_gpu_timing));
+  printf("letkf_gpu_tuning %zu\n", sizeof(letkf_gpu_tuning));
   F(letkf_gpu_dims, precision);
   F(letkf_gpu_dims, path);
   F(letkf_loc_desc, hzloc_sat);
@@ -20,5 +21,7 @@ int main(void) {
   F(letkf_loc_desc, lev_depth);
   F(letkf_gpu_timing, n_launch);
   F(letkf_gpu_timing, order_ms);
+  F(letkf_gpu_tuning, n_tried);
+  F(letkf_gpu_tuning, default_ms);
   return 0;
 }

@@ -38,7 +38,7 @@ def test_struct_layouts(tmp_path):
                .stdout.splitlines())
     mirror = {"letkf_gpu_dims": abi.Dims, "letkf_linearinterp_lat": abi.LinearInterpLat,
               "letkf_loc_desc": abi.LocDesc, "letkf_gpu_diag": abi.Diag,
-              "letkf_gpu_timing": abi.Timing}
+              "letkf_gpu_timing": abi.Timing, "letkf_gpu_tuning": abi.Tuning}
     for name, cls in mirror.items():
         assert C.sizeof(cls) == int(out[name]), name
     for key, val in out.items():

@@ -65,3 +65,36 @@ def test_rational_and_eigen_paths_agree(prec, k, ns, monkeypatch):
     print(prec, k, ns, d, flush=True)
     assert max(d.values()) <= 1e-9, d
     _mixed(p, prec)
+
+
+def test_launch_choices_and_autotune():
+    """The problem order (four modes, bucket edges split 1, 2 or 4 ways per
+    octave) only reorders independent problems: the analysis is bitwise the
+    same under each.  The register and LDS tridiagonalizations agree to
+    rounding.  autotune times them, keeps the fastest and leaves that
+    analysis; bad settings are refused."""
+    from letkf_amd.solver import LetkfError, LetkfGpuSolver
+    p = synth.random_problem(seed=777, k=40, ns=2, nobs=4000, rtps=0.4)
+    with LetkfGpuSolver(p, precision="bf16x3", path="batched") as s:
+        s.run()
+        st0, mn0, sp0 = s.state_f64()
+        for order in range(4):
+            for bb in (range(3) if order >= 2 else [0]):
+                print("order", order, "bucket_bits", bb, flush=True)
+                s.set_tuning(order, bb, 1)
+                s.run()
+                st, mn, sp = s.state_f64()
+                assert np.array_equal(st, st0) and np.array_equal(mn, mn0)
+        s.set_tuning(2, 0, 0)
+        s.run()
+        st, mn, sp = s.state_f64()
+        assert rel(st, st0) <= 1e-12 and rel(mn, mn0) <= 1e-12
+        t = s.autotune(repeats=1)
+        print("autotune", t, flush=True)
+        assert t["n_tried"] == 9 and t["best_ms"] <= t["default_ms"]
+        assert s.tuning()["order"] == t["order"]
+        st, mn, sp = s.state_f64()
+        assert rel(st, st0) <= 1e-12 and rel(mn, mn0) <= 1e-12
+        for bad in ((4, 0, 1), (2, 3, 1), (2, 0, 2)):
+            with pytest.raises(LetkfError):
+                s.set_tuning(*bad)

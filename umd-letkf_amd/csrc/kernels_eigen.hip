@@ -727,13 +727,10 @@ template <int K, int NV>
 static void tridiag_launch(hipStream_t st, unsigned nb, size_t lds, int k, const LocDev& L,
                            const StateDev& S, const VarsDev& V, const ChunkDev& C,
                            const DiagDev& D, double* B, const double* bvec, double* tau,
-                           double* dd, double* ee, double* U, int* nvec, int* status) {
+                           double* dd, double* ee, double* U, int* nvec, int* status,
+                           bool reg) {
   // k >= 32 (K = 64): the register-resident reduction, rows padded to a
-  // multiple of 8 (LETKF_GPU_TRIREG=0 keeps the LDS kernel)
-  static const bool reg = [] {
-    const char* e = getenv("LETKF_GPU_TRIREG");
-    return !(e && atoi(e) == 0);
-  }();
+  // multiple of 8, unless reg is false (the LDS kernel)
   if (K == 64 && reg &&
       launch_tridiag_reg(st, nb, (k + 7) / 8 * 8, NV, k, K, L, S, V, C, D, B, bvec, tau, dd, ee,
                          U, nvec, status))
@@ -753,18 +750,18 @@ static void pipeline(hipStream_t st, const LocDev& L, const StateDev& S, const V
                      const ChunkDev& C, const DiagDev& D, double* B, const double* bvec,
                      double* tau, double* dd, double* ee, double2* rot, int cap_rot,
                      int4* itr, int cap_it, int* nit, int* status, double* Z, int NV,
-                     double* U, int* nvec, const RatTab& R, double* ana) {
+                     double* U, int* nvec, const RatTab& R, double* ana, bool reg) {
   const int64_t np = C.ngp * C.G;
   const int k = (int)S.k;
   const unsigned nb = (unsigned)np;
   const unsigned nl = (unsigned)((np + 63) / 64);
   const size_t lds = sizeof(double) * (size_t)k * (k + 1);
   switch (NV) {
-    case 0: tridiag_launch<K, 0>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status); break;
-    case 2: tridiag_launch<K, 2>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status); break;
-    case 3: tridiag_launch<K, 3>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status); break;
-    case 5: tridiag_launch<K, 5>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status); break;
-    default: tridiag_launch<K, 9>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status); break;
+    case 0: tridiag_launch<K, 0>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg); break;
+    case 2: tridiag_launch<K, 2>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg); break;
+    case 3: tridiag_launch<K, 3>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg); break;
+    case 5: tridiag_launch<K, 5>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg); break;
+    default: tridiag_launch<K, 9>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg); break;
   }
   if (NV > 0 && R.nb > 0) {
     // rational T^(-1/2): no QL, no eigenvectors
@@ -805,14 +802,14 @@ void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
                            double* B, const double* bvec, double* tau, double* dd,
                            double* ee, double2* rot, int cap_rot, int4* itr,
                            int cap_it, int* nit, int* status, double* Z, int NV, double* U,
-                           int* nvec, const RatTab& R, double* ana) {
+                           int* nvec, const RatTab& R, double* ana, bool trireg) {
   if (C.ngp * C.G <= 0) return;
   if (eigen_pipeline_K((int)S.k) == 32)
     pipeline<32>(st, L, S, V, C, D, B, bvec, tau, dd, ee, rot, cap_rot, itr, cap_it, nit,
-                 status, Z, NV, U, nvec, R, ana);
+                 status, Z, NV, U, nvec, R, ana, trireg);
   else
     pipeline<64>(st, L, S, V, C, D, B, bvec, tau, dd, ee, rot, cap_rot, itr, cap_it, nit,
-                 status, Z, NV, U, nvec, R, ana);
+                 status, Z, NV, U, nvec, R, ana, trireg);
 }
 
 }  // namespace lk

@@ -265,10 +265,12 @@ void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev
 }
 
 // order of the problems of a chunk: mode 0 gridpoint order, mode 1 longest
-// first (descending m), mode 2 power-of-two buckets of m, largest first,
-// gridpoint order inside a bucket, mode 3 the same buckets with a Morton
-// (Z-order) walk over a 0.25-degree lat/lon lattice inside a bucket, so that
-// problems in flight together share observation columns in L2
+// first (descending m), mode 2 buckets of m, largest first, gridpoint order
+// inside a bucket, mode 3 the same buckets with a Morton (Z-order) walk over
+// a 0.25-degree lat/lon lattice inside a bucket, so that problems in flight
+// together share observation columns in L2.  Bucket edges: the powers of two
+// split into 2^bbits equal parts (bbits 0..2), so bucket
+// floor(log2 m) . (the next bbits bits of m).
 __device__ __forceinline__ uint32_t spread_bits(uint32_t x) {   // 16 -> 32 bits
   x &= 0xffffu;
   x = (x | (x << 8)) & 0x00ff00ffu;
@@ -278,7 +280,19 @@ __device__ __forceinline__ uint32_t spread_bits(uint32_t x) {   // 16 -> 32 bits
   return x;
 }
 
-__global__ void k_bucket_keys(int mode, int64_t np, int G, int64_t i0,
+__device__ __forceinline__ uint32_t bucket_desc(int64_t m, int bbits) {
+  // 0 for the largest bucket; unsolved problems (m = 0) last
+  const uint32_t nb = 32u << bbits;
+  if (m <= 0) return nb - 1u;
+  const uint32_t L = 63u - (uint32_t)__clzll((unsigned long long)m);   // floor(log2 m)
+  const uint32_t Lc = L > 30u ? 30u : L;
+  const uint32_t mask = (1u << bbits) - 1u;
+  const uint32_t sub = L >= (uint32_t)bbits ? (uint32_t)(m >> (L - bbits)) & mask
+                                            : (uint32_t)(m << (bbits - L)) & mask;
+  return nb - 2u - ((Lc << bbits) | sub);
+}
+
+__global__ void k_bucket_keys(int mode, int bbits, int64_t np, int G, int64_t i0,
                               const int64_t* __restrict__ cnt,
                               const uint8_t* __restrict__ solve,
                               const float* __restrict__ lat, const float* __restrict__ lon,
@@ -287,12 +301,14 @@ __global__ void k_bucket_keys(int mode, int64_t np, int G, int64_t i0,
   if (p >= np) return;
   const int64_t m = solve[p] ? cnt[p] : 0;
   uint32_t key = (uint32_t)p;
-  const uint32_t b = m > 0 ? 64u - (uint32_t)__clzll((unsigned long long)m) : 0u;
-  const uint32_t bucket = 31u - (b > 31u ? 31u : b);
+  // bucket id in the top 5 + bbits bits, the position inside in the rest
+  // (27 - bbits >= 25 bits; a chunk holds at most 2^19 problems)
+  const int low = 27 - bbits;
+  const uint32_t bucket = bucket_desc(m, bbits);
   if (mode == 1) {
     key = 0xffffffffu - (uint32_t)(m > 0xfffffffe ? 0xfffffffe : m);
   } else if (mode == 2) {
-    key = (bucket << 27) | (uint32_t)p;
+    key = (bucket << low) | (uint32_t)p;
   } else if (mode == 3) {
     const int64_t ij = i0 + p / G;
     float fl = (lat[ij] + 90.0f) * 4.0f;
@@ -301,18 +317,18 @@ __global__ void k_bucket_keys(int mode, int64_t np, int G, int64_t i0,
     const uint32_t li = (uint32_t)fminf(fmaxf(fl, 0.0f), 1023.0f);
     const uint32_t lo = (uint32_t)fminf(fmaxf(fo, 0.0f), 2047.0f);
     const uint32_t z = spread_bits(li) | (spread_bits(lo) << 1);   // 22 bits
-    key = (bucket << 27) | (z << 5) | (uint32_t)(p % G & 31);
+    key = (bucket << low) | ((z >> bbits) << 5) | (uint32_t)(p % G & 31);
   }
   keys[p] = key;
   vals[p] = (int32_t)p;
 }
 
-void launch_bucket_keys(hipStream_t st, int mode, int64_t np, int G, int64_t i0,
+void launch_bucket_keys(hipStream_t st, int mode, int bbits, int64_t np, int G, int64_t i0,
                         const int64_t* cnt, const uint8_t* solve, const float* lat,
                         const float* lon, uint32_t* keys, int32_t* vals) {
   if (np <= 0) return;
-  k_bucket_keys<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(mode, np, G, i0, cnt, solve, lat,
-                                                             lon, keys, vals);
+  k_bucket_keys<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(mode, bbits, np, G, i0, cnt, solve,
+                                                             lat, lon, keys, vals);
 }
 
 }  // namespace lk

@@ -11,6 +11,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -112,6 +113,8 @@ struct letkf_gpu_t {
   letkf_gpu_timing timing{};
   int tune_xcd = 0;       // XCD-contiguous block order in the product kernels
   int tune_bucket = 2;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
+  int tune_bbits = 0;     // bucket edges: powers of two split in 2^bbits parts
+  int tune_trireg = 1;    // register-resident tridiagonalization (32 <= k < 64)
   int tune_vec = 1;       // vector path when every group has few slabs
   int tune_rat = 1;       // batched vector path: rational A^(-1/2), no eigensolve
 
@@ -185,7 +188,9 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   h->ns = d->grid_ns;
   h->nij = d->ij_count;
   if (const char* e = getenv("LETKF_GPU_XCD")) h->tune_xcd = atoi(e);
-  if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = atoi(e);
+  if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = std::min(3, std::max(0, atoi(e)));
+  if (const char* e = getenv("LETKF_GPU_BBITS")) h->tune_bbits = std::min(2, std::max(0, atoi(e)));
+  if (const char* e = getenv("LETKF_GPU_TRIREG")) h->tune_trireg = atoi(e) != 0;
   if (const char* e = getenv("LETKF_GPU_VEC")) h->tune_vec = atoi(e);
   if (const char* e = getenv("LETKF_GPU_RAT")) h->tune_rat = atoi(e);
   hipGetDevice(&h->device);
@@ -828,7 +833,7 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     HIPCHK(h, hipGetLastError());
     rec();
     if (h->batched()) {
-      launch_bucket_keys(st, h->tune_bucket, np, (int)G, i0, h->d_cnt, h->d_solve, h->d_lat,
+      launch_bucket_keys(st, h->tune_bucket, h->tune_bbits, np, (int)G, i0, h->d_cnt, h->d_solve, h->d_lat,
                          h->d_lon, h->d_bkey, h->d_bval);
       size_t sb = h->stmp_bytes;
       HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(h->d_stmp, sb, h->d_bkey, h->d_bkey2,
@@ -846,7 +851,8 @@ int letkf_gpu_run(letkf_gpu_t* h) {
       if (h->use_rat) { R.tab = h->d_rat; R.n = h->d_ratn; R.nb = kRatBuckets; }
       launch_eigen_pipeline(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_tau, h->d_dd,
                             h->d_ee, h->d_rot, h->cap_rot, h->d_itr, h->cap_it, h->d_nit,
-                            h->d_status, h->d_Z, h->vec_nv, h->d_U, h->d_nvec, R, h->d_ana);
+                            h->d_status, h->d_Z, h->vec_nv, h->d_U, h->d_nvec, R, h->d_ana,
+                            h->tune_trireg != 0);
       HIPCHK(h, hipGetLastError());
       // problems the QL could not finish go to the Jacobi solver
       launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_status, h->d_ana);
@@ -1050,6 +1056,92 @@ int letkf_gpu_get_selection(letkf_gpu_t* h, int64_t ij, int32_t group, int32_t* 
 int letkf_gpu_get_timing(const letkf_gpu_t* h, letkf_gpu_timing* t) {
   if (!h || !t) return LETKF_EINVAL;
   *t = h->timing;
+  return LETKF_OK;
+}
+
+int letkf_gpu_get_tuning(const letkf_gpu_t* h, letkf_gpu_tuning* out) {
+  if (!h || !out) return LETKF_EINVAL;
+  *out = letkf_gpu_tuning{h->tune_bucket, h->tune_bbits, h->tune_trireg, 0, 0.0, 0.0};
+  return LETKF_OK;
+}
+
+int letkf_gpu_set_tuning(letkf_gpu_t* h, const letkf_gpu_tuning* t) {
+  if (!h) return LETKF_EINVAL;
+  if (!t) return fail(h, LETKF_EINVAL, "null tuning");
+  if (t->order < 0 || t->order > 3) return fail(h, LETKF_EINVAL, "tuning order %d outside 0..3", t->order);
+  if (t->bucket_bits < 0 || t->bucket_bits > 2)
+    return fail(h, LETKF_EINVAL, "tuning bucket_bits %d outside 0..2", t->bucket_bits);
+  if (t->tridiag_reg != 0 && t->tridiag_reg != 1)
+    return fail(h, LETKF_EINVAL, "tuning tridiag_reg %d not 0 or 1", t->tridiag_reg);
+  h->tune_bucket = t->order;
+  h->tune_bbits = t->bucket_bits;
+  h->tune_trireg = t->tridiag_reg;
+  return LETKF_OK;
+}
+
+int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out) {
+  if (!h) return LETKF_EINVAL;
+  if (repeats < 1 || repeats > 16)
+    return fail(h, LETKF_EINVAL, "autotune repeats %d outside 1..16", repeats);
+  letkf_gpu_tuning best{};
+  int rc = letkf_gpu_get_tuning(h, &best);
+  if (rc) return rc;
+  rc = letkf_gpu_run(h);   // allocations and list capacity outside the timings
+  if (rc) return rc;
+  int tried = 0;
+  auto time_of = [&](const letkf_gpu_tuning& t, double* ms) -> int {
+    int r = letkf_gpu_set_tuning(h, &t);
+    if (r) return r;
+    double b = 1e300;
+    for (int i = 0; i < repeats; ++i) {
+      const auto t0 = std::chrono::steady_clock::now();
+      r = letkf_gpu_run(h);
+      if (r) return r;
+      const std::chrono::duration<double, std::milli> dt = std::chrono::steady_clock::now() - t0;
+      b = std::min(b, dt.count());
+    }
+    ++tried;
+    *ms = b;
+    return LETKF_OK;
+  };
+  double best_ms = 0.0;
+  rc = time_of(best, &best_ms);
+  if (rc) return rc;
+  const double default_ms = best_ms;
+  auto consider = [&](letkf_gpu_tuning t) -> int {
+    double ms = 0.0;
+    const int r = time_of(t, &ms);
+    if (r) return r;
+    if (ms < best_ms) { best = t; best_ms = ms; }
+    return LETKF_OK;
+  };
+  // order and bucket edges (the batched product only), then the
+  // tridiagonalization kernel (32 <= k < 64 only)
+  if (h->batched()) {
+    const letkf_gpu_tuning base = best;
+    for (int order = 0; order <= 3; ++order)
+      for (int bb = 0; bb <= (order >= 2 ? 2 : 0); ++bb) {
+        if (order == base.order && bb == base.bucket_bits) continue;
+        letkf_gpu_tuning t = base;
+        t.order = order;
+        t.bucket_bits = bb;
+        if ((rc = consider(t))) return rc;
+      }
+  }
+  if (h->k >= 32 && h->k < 64 && h->eigK == 64) {
+    letkf_gpu_tuning t = best;
+    t.tridiag_reg = 1 - t.tridiag_reg;
+    if ((rc = consider(t))) return rc;
+  }
+  // leave the chosen settings and their analysis in the handle
+  rc = letkf_gpu_set_tuning(h, &best);
+  if (rc) return rc;
+  rc = letkf_gpu_run(h);
+  if (rc) return rc;
+  best.n_tried = tried;
+  best.best_ms = best_ms;
+  best.default_ms = default_ms;
+  if (out) *out = best;
   return LETKF_OK;
 }
 

@@ -183,7 +183,7 @@ void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
                             const double* bvec, const int* only_flagged, double* ana);
 void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev& S,
                        const ChunkDev& C, const int32_t* perm, double* B, double* bvec);
-void launch_bucket_keys(hipStream_t st, int mode, int64_t np, int G, int64_t i0,
+void launch_bucket_keys(hipStream_t st, int mode, int bbits, int64_t np, int G, int64_t i0,
                         const int64_t* cnt, const uint8_t* solve, const float* lat,
                         const float* lon, uint32_t* keys, int32_t* vals);
 // register-resident tridiagonalization for rows of KQ (a multiple of 8,
@@ -215,7 +215,7 @@ void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
                            double* B, const double* bvec, double* tau, double* dd,
                            double* ee, double2* rot, int cap_rot, int4* itr,
                            int cap_it, int* nit, int* status, double* Z, int NV, double* U,
-                           int* nvec, const RatTab& R, double* ana);
+                           int* nvec, const RatTab& R, double* ana, bool trireg);
 void launch_finalize(hipStream_t st, const LocDev& L, const StateDev& S,
                      const VarsDev& V, const ChunkDev& C, const DiagDev& D,
                      float rtps, float rtpp, float mul, const double* ana_in,

@@ -40,6 +40,7 @@ PROGRAM letkf_gpu_driver
   TYPE(letkf_linearinterp_lat) :: tab
   TYPE(letkf_gpu_diag) :: diag
   TYPE(letkf_gpu_timing) :: timing
+  TYPE(letkf_gpu_tuning) :: tuning
   INTEGER(c_int8_t), ALLOCATABLE :: raw(:)
   REAL(c_float), ALLOCATABLE, TARGET :: lev_depth(:)
   REAL(c_float), ALLOCATABLE :: state(:,:,:), mean(:,:), sprd(:,:), lat(:), lon(:)
@@ -55,8 +56,8 @@ PROGRAM letkf_gpu_driver
 
   CALL get_command_argument(1, fin)
   IF (TRIM(fin) == "--sizes") THEN
-     PRINT '(A,5I8)', "sizes", c_sizeof(dims), c_sizeof(tab), c_sizeof(loc), &
-          c_sizeof(diag), c_sizeof(timing)
+     PRINT '(A,6I8)', "sizes", c_sizeof(dims), c_sizeof(tab), c_sizeof(loc), &
+          c_sizeof(diag), c_sizeof(timing), c_sizeof(tuning)
      STOP
   END IF
   CALL get_command_argument(2, fout)

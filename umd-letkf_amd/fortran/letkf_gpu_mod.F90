@@ -79,12 +79,19 @@ MODULE letkf_gpu_mod
      REAL(c_double) :: order_ms
   END TYPE letkf_gpu_timing
 
+  !> letkf_gpu_tuning
+  TYPE, BIND(C), PUBLIC :: letkf_gpu_tuning
+     INTEGER(c_int32_t) :: order, bucket_bits, tridiag_reg, n_tried
+     REAL(c_double) :: best_ms, default_ms
+  END TYPE letkf_gpu_tuning
+
   PUBLIC :: letkf_gpu_version, letkf_gpu_create, letkf_gpu_destroy, letkf_gpu_last_error, &
        letkf_gpu_set_state, letkf_gpu_set_obs, letkf_gpu_set_statevars, &
        letkf_gpu_set_localizer, letkf_gpu_set_solver, letkf_gpu_run, letkf_gpu_get_state, &
        letkf_gpu_get_state_f64, letkf_gpu_get_diag, letkf_gpu_maxgroups, &
        letkf_gpu_get_selection, letkf_gpu_get_timing, letkf_gpu_errmsg, &
-       letkf_gpu_set_state_members, letkf_gpu_set_obs_members, letkf_gpu_get_state_members
+       letkf_gpu_set_state_members, letkf_gpu_set_obs_members, letkf_gpu_get_state_members, &
+       letkf_gpu_autotune, letkf_gpu_get_tuning, letkf_gpu_set_tuning
 
   INTERFACE
      FUNCTION letkf_gpu_version() BIND(C, name="letkf_gpu_version")
@@ -238,6 +245,28 @@ MODULE letkf_gpu_mod
        REAL(c_float), INTENT(out) :: members(*), mean_ij(*), sprd_ij(*)
        INTEGER(c_int) :: letkf_gpu_get_state_members
      END FUNCTION letkf_gpu_get_state_members
+
+     FUNCTION letkf_gpu_autotune(h, repeats, t) BIND(C, name="letkf_gpu_autotune")
+       IMPORT :: c_int, c_ptr, c_int32_t, letkf_gpu_tuning
+       TYPE(c_ptr), VALUE :: h
+       INTEGER(c_int32_t), VALUE :: repeats
+       TYPE(letkf_gpu_tuning), INTENT(out) :: t
+       INTEGER(c_int) :: letkf_gpu_autotune
+     END FUNCTION letkf_gpu_autotune
+
+     FUNCTION letkf_gpu_get_tuning(h, t) BIND(C, name="letkf_gpu_get_tuning")
+       IMPORT :: c_int, c_ptr, letkf_gpu_tuning
+       TYPE(c_ptr), VALUE :: h
+       TYPE(letkf_gpu_tuning), INTENT(out) :: t
+       INTEGER(c_int) :: letkf_gpu_get_tuning
+     END FUNCTION letkf_gpu_get_tuning
+
+     FUNCTION letkf_gpu_set_tuning(h, t) BIND(C, name="letkf_gpu_set_tuning")
+       IMPORT :: c_int, c_ptr, letkf_gpu_tuning
+       TYPE(c_ptr), VALUE :: h
+       TYPE(letkf_gpu_tuning), INTENT(in) :: t
+       INTEGER(c_int) :: letkf_gpu_set_tuning
+     END FUNCTION letkf_gpu_set_tuning
 
      FUNCTION c_strlen(s) BIND(C, name="strlen")
        IMPORT :: c_ptr, c_size_t

@@ -80,6 +80,12 @@ class Timing(C.Structure):
                 ("n_launch", C.c_int64), ("order_ms", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("order", C.c_int32), ("bucket_bits", C.c_int32),
+                ("tridiag_reg", C.c_int32), ("n_tried", C.c_int32),
+                ("best_ms", C.c_double), ("default_ms", C.c_double)]
+
+
 EXPORTED = [
     "letkf_gpu_version", "letkf_gpu_create", "letkf_gpu_destroy",
     "letkf_gpu_last_error", "letkf_gpu_set_state", "letkf_gpu_set_obs",
@@ -88,7 +94,8 @@ EXPORTED = [
     "letkf_gpu_get_state_f64", "letkf_gpu_get_diag", "letkf_gpu_maxgroups",
     "letkf_gpu_get_selection", "letkf_gpu_get_timing",
     "letkf_gpu_set_state_members", "letkf_gpu_set_obs_members",
-    "letkf_gpu_get_state_members",
+    "letkf_gpu_get_state_members", "letkf_gpu_autotune", "letkf_gpu_get_tuning",
+    "letkf_gpu_set_tuning",
 ]
 
 _FP = C.POINTER(C.c_float)
@@ -119,6 +126,9 @@ _SIGS = {
     "letkf_gpu_set_obs_members": (C.c_int, [_H, C.c_int64, _FP, _FP, _FP, _FP, _FP,
                                             _FP, _IP, _IP, _IP, _FP]),
     "letkf_gpu_get_state_members": (C.c_int, [_H, _FP, _FP, _FP]),
+    "letkf_gpu_autotune": (C.c_int, [_H, C.c_int32, C.POINTER(Tuning)]),
+    "letkf_gpu_get_tuning": (C.c_int, [_H, C.POINTER(Tuning)]),
+    "letkf_gpu_set_tuning": (C.c_int, [_H, C.POINTER(Tuning)]),
 }
 
 _lib = None

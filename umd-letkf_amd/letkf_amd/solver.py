@@ -185,6 +185,22 @@ class LetkfGpuSolver:
         self._chk(self.lib.letkf_gpu_get_timing(self.h, C.byref(t)))
         return {f: getattr(t, f) for f, _ in abi.Timing._fields_}
 
+    def autotune(self, repeats: int = 2) -> dict:
+        """Time the launch choices on the current inputs and keep the
+        fastest (letkf_gpu_autotune)."""
+        t = abi.Tuning()
+        self._chk(self.lib.letkf_gpu_autotune(self.h, int(repeats), C.byref(t)))
+        return {f: getattr(t, f) for f, _ in abi.Tuning._fields_}
+
+    def tuning(self) -> dict:
+        t = abi.Tuning()
+        self._chk(self.lib.letkf_gpu_get_tuning(self.h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in abi.Tuning._fields_}
+
+    def set_tuning(self, order: int, bucket_bits: int = 0, tridiag_reg: int = 1):
+        t = abi.Tuning(order, bucket_bits, tridiag_reg, 0, 0.0, 0.0)
+        self._chk(self.lib.letkf_gpu_set_tuning(self.h, C.byref(t)))
+
     def final(self):
         if self.h:
             self.lib.letkf_gpu_destroy(self.h)
