@@ -117,6 +117,9 @@ struct letkf_gpu_t {
   int tune_trireg = 1;    // register-resident tridiagonalization (32 <= k < 64)
   int tune_vec = 1;       // vector path when every group has few slabs
   int tune_rat = 1;       // batched vector path: rational A^(-1/2), no eigensolve
+  int tune_mat = 1;       // batched, 64 <= k <= 128: explicit T applied on MFMA
+  bool use_mat = false;
+  int alloc_mat = -1;
 
   bool batched() const {
     return dims.path == LETKF_PATH_BATCHED && dims.precision != LETKF_PREC_FP64;
@@ -193,6 +196,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   if (const char* e = getenv("LETKF_GPU_TRIREG")) h->tune_trireg = atoi(e) != 0;
   if (const char* e = getenv("LETKF_GPU_VEC")) h->tune_vec = atoi(e);
   if (const char* e = getenv("LETKF_GPU_RAT")) h->tune_rat = atoi(e);
+  if (const char* e = getenv("LETKF_GPU_MAT")) h->tune_mat = atoi(e);
   hipGetDevice(&h->device);
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
@@ -674,11 +678,16 @@ static int ensure_outputs(letkf_gpu_t* h) {
   // problem (reference solver for tests); default: tridiagonal QL pipeline
   const char* ev = getenv("LETKF_GPU_EIGEN");
   const bool force_jacobi = ev && strcmp(ev, "jacobi") == 0;
-  const int eigK = force_jacobi ? 0 : eigen_pipeline_K((int)k);
+  // 64 <= k <= 128: explicit transform matrix, applied on MFMA (batched)
+  // or in FP64 (plain); LETKF_GPU_MAT=0 keeps the Jacobi solver (k <= 94)
+  const bool use_mat = !force_jacobi && h->tune_mat && transform_mat_supported((int)k);
+  const int eigK = (force_jacobi || use_mat) ? 0 : eigen_pipeline_K((int)k);
   const int nv = (eigK && h->tune_vec) ? eigen_vec_nv(max_slabs_per_group(h)) : 0;
-  if (h->alloc_G == G && h->eigK == eigK && h->alloc_nv == nv) return LETKF_OK;
+  if (h->alloc_G == G && h->eigK == eigK && h->alloc_nv == nv && h->alloc_mat == (int)use_mat)
+    return LETKF_OK;
   h->eigK = eigK;
   h->vec_nv = nv;
+  h->use_mat = use_mat;
   HIPCHK(h, dalloc(&h->d_lgm, (size_t)(G * nij)));
   HIPCHK(h, dalloc(&h->d_lgloc, (size_t)(G * nij)));
   HIPCHK(h, dalloc(&h->d_lghash, (size_t)(G * nij)));
@@ -732,7 +741,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
     HIPCHK(h, dalloc((char**)&h->d_stmp, sb));
     h->stmp_bytes = sb;
   }
-  if (h->use_rat && !h->d_rat) {
+  if ((h->use_rat || h->use_mat) && !h->d_rat) {
     // x^(-1/2) on [1, 2^b]: the fewest nodes (<= 31) for 1e-12 relative error,
     // else 31 nodes
     std::vector<double> tab((size_t)kRatBuckets * 64, 0.0);
@@ -750,6 +759,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
   }
   h->alloc_G = G;
   h->alloc_nv = nv;
+  h->alloc_mat = (int)use_mat;
   return LETKF_OK;
 }
 
@@ -765,7 +775,10 @@ int letkf_gpu_run(letkf_gpu_t* h) {
                 "has %d", h->loc.max_var_lev, h->loc.nlev);
   if (h->ns > 2048)
     return fail(h, LETKF_EUNSUPPORTED, "grid_ns %lld above 2048", (long long)h->ns);
-  if (eigen_lds_bytes((int)h->k) + 8192 + 64 > 150 * 1024)
+  const char* evj = getenv("LETKF_GPU_EIGEN");
+  const bool mat = !(evj && strcmp(evj, "jacobi") == 0) && h->tune_mat &&
+                   transform_mat_supported((int)h->k);
+  if (!mat && eigen_lds_bytes((int)h->k) + 8192 + 64 > 150 * 1024)
     return fail(h, LETKF_EUNSUPPORTED, "ens_size %lld too large for the FP64 eigensolver",
                 (long long)h->k);
   if (h->batched() && h->nobs * (int64_t)h->kp * 4 >= (1ll << 31))
@@ -846,7 +859,11 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     }
     HIPCHK(h, hipGetLastError());
     rec();
-    if (h->eigK) {
+    if (h->use_mat) {
+      const RatTab R{h->d_rat, h->d_ratn, kRatBuckets};
+      launch_transform_mat(st, h->batched() ? h->dims.precision : LETKF_PREC_FP64, h->loc, S,
+                           V, C, D, h->d_B, h->d_b, h->d_status, R, h->d_ana);
+    } else if (h->eigK) {
       RatTab R;
       if (h->use_rat) { R.tab = h->d_rat; R.n = h->d_ratn; R.nb = kRatBuckets; }
       launch_eigen_pipeline(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_tau, h->d_dd,

@@ -216,6 +216,11 @@ void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
                            double* ee, double2* rot, int cap_rot, int4* itr,
                            int cap_it, int* nit, int* status, double* Z, int NV, double* U,
                            int* nvec, const RatTab& R, double* ana, bool trireg);
+// explicit transform matrix and its MFMA application (64 <= k <= 128)
+bool transform_mat_supported(int k);
+void launch_transform_mat(hipStream_t st, int prec, const LocDev& L, const StateDev& S,
+                          const VarsDev& V, const ChunkDev& C, const DiagDev& D, double* B,
+                          const double* bvec, int* status, const RatTab& R, double* ana);
 void launch_finalize(hipStream_t st, const LocDev& L, const StateDev& S,
                      const VarsDev& V, const ChunkDev& C, const DiagDev& D,
                      float rtps, float rtpp, float mul, const double* ana_in,

@@ -1,0 +1,12 @@
+// transform_mat_nr48.hip -- k_transform_mat for rows of 48 per half
+// (2 x 48 >= k); one instantiation per file keeps the build parallel.
+#include "transform_mat.h"
+
+namespace lk {
+
+void launch_transform_nr48(hipStream_t st, unsigned nb, int k, const ChunkDev& C, double* B,
+                            const double* bvec, int* status, const RatTab& R) {
+  launch_transform_nr<48>(st, nb, k, C, B, bvec, status, R);
+}
+
+}  // namespace lk

@@ -375,6 +375,131 @@ def config2(seed=0x1E7F0002, nx=360, ny=180, nlev=50, k=40, nprof=4000,
                         meta=dict(config=2, nx=nx, ny=ny, nlev=nlev, seed=seed))
 
 
+# ---------------------------------------------------------------- config 4
+C4_ERR = (0.2, 0.02, 0.05, 0.05)           # T (K), S (psu), u, v (m/s)
+C4_SIGMA = (0.5, 0.05, 0.1, 0.1)           # background perturbation scale
+# C2's radii read as cut-off radii, as for C3: L = radius / sqrt(40/3)
+C4_L = (700e3 / 3.6514837741851807, 200e3 / 3.6514837741851807)
+
+
+def config4(seed=0x1E7F0004, nx=720, ny=360, nlev=75, k=100, nprof=13000, nsurf=25000,
+            modes=32, hcorr=500e3, rtps=0.6, chunk=4096):
+    """Config 4: 0.5-degree 720x360 grid, 75 levels, T, S, u, v (ns = 300),
+    k = 100, FP32 state (31.1 GB at full size), 13,000 profiles x 75 levels
+    (T or S) plus 25,000 surface u or v points (1M obs), per-variable errors
+    0.2 K, 0.02 psu, 0.05 m/s, LOC_NOVRT with 700 km (equator) to 200 km
+    (poles) cut-off radii, RTPS 0.6.
+
+    Perturbations are low rank and separable, so that the 31 GB background
+    is written once, straight into the reference layout: member e of
+    variable v at level l is sigma_v sum_r C_v[e, r] Z_v[r, l] F[r, :] with
+    `modes` smooth horizontal fields F (Gaussian correlation hcorr), smooth
+    random vertical modes Z_v and N(0, 1) coefficients C_v.  The background
+    is mean = base + mean over members, perturbations = members - mean, in
+    FP32.  hx is the bilinear interpolation of each member (base + the same
+    modes interpolated, since interpolation is linear); the truth is one
+    more coefficient draw; val = H(truth) + N(0, err_v^2)."""
+    g = rng(seed)
+    lon, lat, lon_ij, lat_ij = flat_grid(nx, ny)
+    nij = nx * ny
+    ns = 4 * nlev
+    depth = 5000.0 * (np.arange(nlev) / max(nlev - 1, 1)) ** 2
+    mask = land_mask(lat_ij, lon_ij)
+    cl = np.cos(np.radians(lat_ij))
+    F = smooth_perts(g, modes, lat, lon, hcorr, 1.0).reshape(modes, nij).astype(np.float32)
+    # vertical modes: Gaussian bumps of random centre, width and sign,
+    # normalised to unit variance at every level
+    Z = np.zeros((4, modes, nlev))
+    for v in range(4):
+        z0 = g.uniform(0.0, 5000.0, modes)
+        wz = g.uniform(150.0, 1500.0, modes)
+        Z[v] = np.exp(-0.5 * ((depth[None, :] - z0[:, None]) / wz[:, None]) ** 2)
+        Z[v] *= g.choice([-1.0, 1.0], modes)[:, None]
+        Z[v] /= np.sqrt((Z[v] ** 2).sum(0, keepdims=True))
+    Cv = g.standard_normal((4, k + 1, modes))          # member k is the truth
+    sig = np.array(C4_SIGMA)
+
+    def base(v, lev, ij):
+        z = depth[lev][:, None]
+        c = cl[ij][None, :] if np.ndim(ij) else cl[ij]
+        if v == 0:
+            return (2.0 + 18.0 * np.exp(-z / 800.0)) * (0.6 + 0.4 * c)
+        if v == 1:
+            return 34.5 + 0.5 * np.exp(-z / 1000.0) * c
+        if v == 2:
+            return 0.2 * c * np.exp(-z / 500.0)
+        return 0.05 * np.sin(2.0 * np.radians(lon_ij[ij]))[None, :] * np.exp(-z / 500.0)
+
+    state = np.empty((k, ns, nij), np.float32, order="F")
+    mean = np.empty((ns, nij), np.float32, order="F")
+    sprd = np.empty((ns, nij), np.float32, order="F")
+    lev = np.arange(nlev)
+    for i0 in range(0, nij, chunk):
+        ij = np.arange(i0, min(nij, i0 + chunk))
+        for v in range(4):
+            H = (Z[v][:, :, None] * F[None, :, ij].transpose(1, 0, 2)).astype(np.float32)
+            P = (Cv[v, :k].astype(np.float32) @ H.reshape(modes, -1)).reshape(k, nlev, ij.size)
+            P *= np.float32(sig[v])
+            pm = P.sum(0, dtype=np.float32) / np.float32(k)
+            P -= pm[None]
+            sl = slice(v * nlev, (v + 1) * nlev)
+            state[:, sl, i0:i0 + ij.size] = P
+            mean[sl, i0:i0 + ij.size] = (base(v, lev, ij) + pm).astype(np.float32)
+            sprd[sl, i0:i0 + ij.size] = np.sqrt((P * P).sum(0, dtype=np.float32) /
+                                                np.float32(k - 1))
+    # observations: profiles (T for even, S for odd) and surface u / v
+    def ocean_points(n):
+        a, b = np.zeros(0), np.zeros(0)
+        while a.size < n:
+            x, y = uniform_sphere(g, 2 * n)
+            keep = ~land_mask(x, y)
+            a, b = np.concatenate([a, x[keep]]), np.concatenate([b, y[keep]])
+        return a[:n], b[:n]
+    plat0, plon0 = ocean_points(nprof)
+    slat, slon = ocean_points(nsurf)
+    pvar = np.concatenate([np.repeat(np.arange(nprof) % 2, nlev), 2 + np.arange(nsurf) % 2])
+    plev = np.concatenate([np.tile(lev, nprof), np.zeros(nsurf, np.int64)])
+    olat = np.concatenate([np.repeat(plat0, nlev), slat])
+    olon = np.concatenate([np.repeat(plon0, nlev), slon])
+    nobs = olat.size
+    HF = bilinear(F.reshape(modes, ny, nx).astype(np.float64), lat, lon, olat, olon)  # (modes, nobs)
+    hx = np.zeros((k, nobs))
+    ht = np.zeros(nobs)
+    for v in range(4):
+        sel = np.nonzero(pvar == v)[0]
+        if sel.size == 0:
+            continue
+        Hs = Z[v][:, plev[sel]] * HF[:, sel]                       # (modes, nsel)
+        dv = sig[v] * (Cv[v] @ Hs)                                 # (k + 1, nsel)
+        hb = np.zeros(sel.size)
+        for l in np.unique(plev[sel]):
+            q = sel[plev[sel] == l]
+            bf = base(v, np.array([l]), np.arange(nij)).reshape(ny, nx)
+            hb[plev[sel] == l] = bilinear(bf, lat, lon, olat[q], olon[q])
+        hx[:, sel] = hb[None] + dv[:k]
+        ht[sel] = hb + dv[k]
+    err = np.array(C4_ERR)[pvar]
+    val = ht + g.normal(0.0, 1.0, nobs) * err
+    hxp, hxm = obs_from_hx(hx)
+    loc = Localizer(cls="LOC_NOVRT", hzloc=LinearInterpLat([0.0, 90.0], list(C4_L)))
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    p = Problem(k=k, ns=ns, nij=nij, state=state, mean=mean, sprd=sprd,
+                lat=f32(lat_ij), lon=f32(lon_ij), mask=np.ascontiguousarray(mask, np.int32),
+                olat=f32(olat), olon=f32(olon), ozdim=f32(depth[plev]), otime=f32(np.zeros(nobs)),
+                oval=f32(val), oerr=f32(err ** 2),
+                obsid=np.ascontiguousarray(pvar + 1, np.int32),
+                platid=np.ascontiguousarray(np.where(pvar < 2, 1, 2), np.int32),
+                hx=hxp, hxmean=hxm,
+                var_s=np.ascontiguousarray([0, nlev, 2 * nlev, 3 * nlev], np.int32),
+                var_lev=np.ascontiguousarray([nlev] * 4, np.int32),
+                inc_max=f32(np.full(4, 9.99e36)),
+                bounds=np.asfortranarray(np.array([[-9.99e36] * 4, [9.99e36] * 4]), np.float32),
+                loc=loc, rtps=float(rtps), rtpp=0.0, mul=1.0,
+                meta=dict(config=4, nx=nx, ny=ny, nlev=nlev, seed=seed))
+    p.validate()
+    return p
+
+
 # ---------------------------------------------------------------- simple3d
 SIMPLE3D_OBS = {
     # ob_type, plat_type, state_var, lat, lon, depth, time, val (O-F), err
