@@ -14,7 +14,9 @@ sample of gridpoints, all host threads up to 16) and the error of the GPU
 analysis against that oracle on the same sample.
 
 --config c5 runs the cycled analysis instead (ten cycles on the same grid,
-fresh observations each cycle) and prints per-cycle latency percentiles.
+fresh observations each cycle) and prints per-cycle latency percentiles;
+--config c4 the 0.5-degree 75-level four-variable analysis at k = 100 with
+the error of every precision against the oracle.
 """
 import argparse
 import dataclasses
@@ -269,6 +271,90 @@ def bench_cycles(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_c4(args):
+    """The 0.5-degree 720x360 x 75-level T/S/u/v analysis (ns = 300, k = 100,
+    31.1 GB FP32 state, 1M obs): the batched path timed at args.precision,
+    the plain FP64 path timed beside it, and the analysis mean and spread of
+    every precision (FP64 plain, FP32, bf16x3, BF16) against the FP64
+    oracle on a random sample of unmasked gridpoints.  One handle at a time
+    (each holds the state, the FP64 analysis and the chunk buffers, about
+    105 GB of HBM)."""
+    import oracle as O
+    from letkf_amd import synth
+    from letkf_amd.solver import LetkfGpuSolver
+
+    nt = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.time()
+    prob = synth.config4()
+    log(f"[c4] inputs: nij={prob.nij} ns={prob.ns} k={prob.k} nobs={prob.nobs} "
+        f"({time.time() - t0:.1f} s)")
+    g = np.random.default_rng(4444)
+    pool = np.nonzero(prob.mask == 0)[0]
+    pts = np.sort(g.choice(pool, min(args.cycle_sample, pool.size), replace=False))
+    t0 = time.time()
+    orc = O.analysis(prob, points=pts, nthreads=nt)
+    log(f"[c4] oracle on {pts.size} gridpoints: {time.time() - t0:.1f} s")
+
+    def rel(a, b):
+        den = np.abs(b).max()
+        return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
+
+    def run(prec, path, steps):
+        tu = time.perf_counter()
+        s = LetkfGpuSolver(prob, precision=prec, path=path)
+        up = time.perf_counter() - tu
+        try:
+            s.run()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                s.run()
+            dt = (time.perf_counter() - t1) / max(steps, 1)
+            tm = s.timing()
+            _, mn, sp = s.state_f64(members=False)
+            d = s.diag()
+        finally:
+            s.final()
+        err = {"mean_rel": rel(mn[:, pts], orc["mean"]), "sprd_rel": rel(sp[:, pts], orc["sprd"]),
+               "mean_maxabs": float(np.abs(mn[:, pts] - orc["mean"]).max()),
+               "selection_bit_exact": bool(np.array_equal(d["lg_selhash"][pts, :].T,
+                                                          orc["lg_hash"]))}
+        res = {"ms_per_step": round(1e3 * dt, 2), "gridpoints_per_s": round(prob.nij / dt, 1),
+               "upload_s": round(up, 2),
+               "phases_ms": {"search": round(tm["search_ms"], 2), "order": round(tm["order_ms"], 2),
+                             "product": round(tm["product_ms"], 2),
+                             "transform": round(tm["eigen_ms"], 2),
+                             "finalize": round(tm["finalize_ms"], 2)},
+               "work": {"solves": int(tm["n_problems"]), "kept_pairs": int(tm["n_pairs"])},
+               "error_vs_fp64_oracle": err}
+        log(f"[c4] {path} {prec}: {json.dumps(res)}")
+        return res, tm
+
+    main_res, tm = run(args.precision, "batched", args.steps)
+    out = {"metric": "analysis gridpoints/sec, 0.5deg 75-level T/S/u/v, k=100, batched "
+                     "mixed-prec vs FP64 GPU; rel err vs FP64 oracle",
+           "value": main_res["gridpoints_per_s"], "unit": "gridpoints/s", "n_gpus": 1,
+           "steps": args.steps, "ms_per_step": main_res["ms_per_step"],
+           "higher_is_better": True, "dtype": args.precision, "data": "synthetic (seeded Philox)",
+           "config": {"workload": "C4: 0.5deg 720x360x75 T/S/u/v (ns=300), k=100, 1M obs "
+                                  "(13,000 profiles x 75 + 25,000 surface), LOC_NOVRT 700->200 km "
+                                  "cut-off, RTPS 0.6", "n_gp": prob.nij, "ns": prob.ns,
+                      "k": prob.k, "n_obs": prob.nobs, "parallelism": "single GPU"},
+           "roofline": roofline(tm, prob.k, "batched", args.precision, "c4"),
+           "batched": main_res, "oracle_sample": int(pts.size)}
+    others = {}
+    for prec in ("fp32", "bf16x3", "bf16"):
+        if prec != args.precision:
+            others[prec], _ = run(prec, "batched", 1)
+    plain, _ = run("fp64", "plain", max(1, min(args.steps, 2)))
+    out["fp64_plain"] = plain
+    out["speedup_vs_fp64_plain"] = round(main_res["gridpoints_per_s"] / plain["gridpoints_per_s"], 3)
+    out["errors_by_precision"] = {p: r["error_vs_fp64_oracle"] for p, r in
+                                  [("fp64_plain", plain), (args.precision, main_res),
+                                   *others.items()]}
+    out["ms_by_precision"] = {p: r["ms_per_step"] for p, r in others.items()}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,7 +377,8 @@ def main():
                     help="keep the default launch choices instead of letkf_gpu_autotune")
     ap.add_argument("--cycles", type=int, default=10, help="c5: number of analysis cycles")
     ap.add_argument("--cycle-sample", type=int, default=1000,
-                    help="c5: gridpoints per cycle checked against the FP64 oracle")
+                    help="c5: gridpoints per cycle checked against the FP64 oracle; "
+                         "c4: gridpoints checked")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, args.config, args.path, args.precision,
@@ -304,6 +391,9 @@ def main():
 
     if args.config == "c5":
         bench_cycles(args)
+        return
+    if args.config == "c4":
+        bench_c4(args)
         return
 
     from letkf_amd.solver import LetkfGpuSolver

@@ -123,14 +123,16 @@ class LetkfGpuSolver:
         self._chk(self.lib.letkf_gpu_maxgroups(self.h, C.byref(g)))
         return g.value
 
-    def state_f64(self):
+    def state_f64(self, members: bool = True):
+        """FP64 analysis (letkf_gpu_get_state_f64); members=False fetches
+        only the mean and spread (state returned as None)."""
         p = self.prob
-        st = np.zeros((p.k, p.ns, p.nij), order="F")
+        st = np.zeros((p.k, p.ns, p.nij), order="F") if members else None
         mn = np.zeros((p.ns, p.nij), order="F")
         sp = np.zeros((p.ns, p.nij), order="F")
         D = C.c_double
-        self._chk(self.lib.letkf_gpu_get_state_f64(self.h, abi.ptr(st, D), abi.ptr(mn, D),
-                                                   abi.ptr(sp, D)))
+        self._chk(self.lib.letkf_gpu_get_state_f64(
+            self.h, abi.ptr(st, D) if members else None, abi.ptr(mn, D), abi.ptr(sp, D)))
         return st, mn, sp
 
     def state_members(self):

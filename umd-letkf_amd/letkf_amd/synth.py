@@ -434,19 +434,22 @@ def config4(seed=0x1E7F0004, nx=720, ny=360, nlev=75, k=100, nprof=13000, nsurf=
     mean = np.empty((ns, nij), np.float32, order="F")
     sprd = np.empty((ns, nij), np.float32, order="F")
     lev = np.arange(nlev)
+    Ct = [np.ascontiguousarray(Cv[v, :k].T * sig[v], np.float32) for v in range(4)]  # (modes, k)
+    Zt = [np.ascontiguousarray(Z[v].T, np.float32) for v in range(4)]                 # (nlev, modes)
     for i0 in range(0, nij, chunk):
         ij = np.arange(i0, min(nij, i0 + chunk))
+        n = ij.size
+        Fi = np.ascontiguousarray(F[:, ij].T)                                        # (n, modes)
         for v in range(4):
-            H = (Z[v][:, :, None] * F[None, :, ij].transpose(1, 0, 2)).astype(np.float32)
-            P = (Cv[v, :k].astype(np.float32) @ H.reshape(modes, -1)).reshape(k, nlev, ij.size)
-            P *= np.float32(sig[v])
-            pm = P.sum(0, dtype=np.float32) / np.float32(k)
-            P -= pm[None]
+            # P[i, l, e]: member fastest, the order of the state's columns
+            H = Fi[:, None, :] * Zt[v][None, :, :]                                   # (n, nlev, modes)
+            P = (H.reshape(n * nlev, modes) @ Ct[v]).reshape(n, nlev, k)
+            pm = P.sum(2, dtype=np.float32) / np.float32(k)                          # (n, nlev)
+            P -= pm[:, :, None]
             sl = slice(v * nlev, (v + 1) * nlev)
-            state[:, sl, i0:i0 + ij.size] = P
-            mean[sl, i0:i0 + ij.size] = (base(v, lev, ij) + pm).astype(np.float32)
-            sprd[sl, i0:i0 + ij.size] = np.sqrt((P * P).sum(0, dtype=np.float32) /
-                                                np.float32(k - 1))
+            state[:, sl, i0:i0 + n] = P.transpose(2, 1, 0)
+            mean[sl, i0:i0 + n] = (base(v, lev, ij) + pm.T).astype(np.float32)
+            sprd[sl, i0:i0 + n] = np.sqrt(np.einsum("ile,ile->li", P, P) / np.float32(k - 1))
     # observations: profiles (T for even, S for odd) and surface u / v
     def ocean_points(n):
         a, b = np.zeros(0), np.zeros(0)
