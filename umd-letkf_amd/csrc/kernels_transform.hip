@@ -1,46 +1,128 @@
-// kernels_transform.hip -- the LETKF transform as an explicit k x k matrix
-// and its application to every slab of the group on MFMA: the batched path
-// for large ensembles (64 <= k <= 128), where a group can hold hundreds of
-// slabs (src/letkf_core.F90:100-141, src/letkf_solver.F90:284-290):
+// kernels_transform.hip -- the large-ensemble transform (64 <= k <= 128):
+// the rational step of the explicit transform matrix and the application of
+// T to every slab of the group (src/letkf_solver.F90:284-290):
 //
-//   T = W + 1 wbar^T,  W = sqrt(k-1) A^(-1/2),  wbar = A^-1 Yb^T R^-1 d,
-//   A = (k-1) I + Yb^T R^-1 Yb,      x_a = T x_b for each slab x_b.
-//
-// k_transform_mat  one 256-thread block per problem.  A, padded to
-//                  N = 2 NR, lives in registers: thread (c, h) holds column
-//                  c, rows [h NR, h NR + NR) (c = lane + 64 (wave & 1),
-//                  h = wave >> 1).
-//   1. Householder reduction A = Q Tri Q^T (LAPACK dsytd2, lower) by
-//      two-sided rank-2 updates A -= v w^T + w v^T; reflector j is stored in
-//      row j of B, tau and the tridiagonal in LDS.
-//   2. M = sqrt(k-1) Tri^(-1/2) = sum_n c_n (Tri + s_n I)^-1 with the
-//      rational nodes of the node table (the Gershgorin bound of Tri picks
-//      the table).  Column c of (Tri + s I)^-1 follows from the top-down and
-//      bottom-up LDL^T pivots D+, D- of Tri + s I: x_c = 1/g_c with
-//      g_c = D+_c + D-_c - (d_c + s), x_i = -(e_i / D+_i) x_(i+1) above c and
-//      x_i = -(e_(i-1) / D-_i) x_(i-1) below it.  Each thread runs these
-//      two-term recurrences for its own column over its own half of the
-//      rows; the values at the half boundary pass through LDS.
-//   3. W = Q M Q^T: the reflectors applied from both sides in reverse order,
-//      the same two-sided rank-2 update as step 1.
-//   4. wbar = W (W b) / (k-1); T = W + 1 wbar^T written over B, row major.
-// k_apply_mat      one 256-thread block per problem; wave w computes members
+//  k_mat_rational  the lower triangle of M = sqrt(k-1) Tri^(-1/2) (see
+//                  transform_mat.h for the three-kernel scheme);
+//  k_apply_mat     one 256-thread block per problem; wave w computes members
 //                  32w..32w+31 of x_a = T x_b for 32 slabs at a time on
 //                  MFMA (32x32x2 FP32, or 32x32x16 BF16 / bf16x3 with FP32
-//                  accumulation) and writes the FP64 analysis.
+//                  accumulation) and writes the FP64 analysis;
+//  k_apply_mat_f64 the same product in FP64 on the vector units (the plain
+//                  path).
 #include <hip/hip_runtime.h>
 
 #include "letkf_internal.h"
 #include "mx_common.h"
+#include "transform_mat.h"
 
 namespace lk {
 
-// k_transform_mat<NR> launchers (transform_mat.h, one file per NR)
-#define LK_NR(n)                                                                         \
-  void launch_transform_nr##n(hipStream_t st, unsigned nb, int k, const ChunkDev& C,     \
-                              double* B, const double* bvec, int* status, const RatTab& R);
+// k_mat_tridiag / k_mat_back launchers (one file per NR)
+#define LK_NR(n)                                                                          \
+  void launch_mat_tridiag_nr##n(hipStream_t st, unsigned nb, int k, const ChunkDev& C,    \
+                                double* B, double* tau, double* dd, double* ee);          \
+  void launch_mat_back_nr##n(hipStream_t st, unsigned nb, int k, const ChunkDev& C,       \
+                             double* B, const double* bvec, const double* tau);
 LK_NR(40) LK_NR(48) LK_NR(56) LK_NR(64)
 #undef LK_NR
+
+// M below the diagonal, one thread per column c; the rows are visited in the
+// same order by every lane (x starts at g_c in row c, is 0 above it), so the
+// pivot reads are broadcasts and the stores of a row are contiguous
+constexpr int kMaxNodes = 32;
+
+// rows rfirst..k-1 of column c of M for the first NN nodes (nn <= NN)
+template <int NN>
+__device__ __forceinline__ void mat_rational_rows(int k, int c, int rfirst, int nn, double m,
+                                                  const double* __restrict__ wn,
+                                                  const double (*Us)[128],
+                                                  const double (*Gs)[128], double* Bp) {
+  double cf[NN], g[NN], x[NN];
+#pragma unroll
+  for (int n = 0; n < NN; ++n) {
+    cf[n] = n < nn ? m * wn[n] : 0.0;
+    g[n] = Gs[n][c];
+    x[n] = 0.0;
+  }
+  for (int r = rfirst; r < k; ++r) {
+    double acc = 0.0;
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+      x[n] = r == c ? g[n] : -Us[n][r] * x[n];
+      acc = fma(cf[n], x[n], acc);
+    }
+    if (r >= c) Bp[(int64_t)r * k + c] = acc;
+  }
+}
+__global__ __launch_bounds__(128) void k_mat_rational(int k, ChunkDev C, double* __restrict__ B,
+                                                      const double* __restrict__ dd,
+                                                      const double* __restrict__ ee,
+                                                      const double* __restrict__ tab,
+                                                      const int* __restrict__ tabn,
+                                                      int nbucket) {
+  __shared__ double sd[128], se[128];
+  __shared__ double Us[kMaxNodes][128], Gs[kMaxNodes][128];
+  __shared__ double sgm[2];
+  const int64_t pair = blockIdx.x;
+  if (!C.solve[pair]) return;
+  const int tid = threadIdx.x, l = tid & 63;
+  const double m = (double)(k - 1);
+  sd[tid] = tid < k ? dd[pair * kMatStride + tid] : 0.0;
+  se[tid] = tid < k - 1 ? ee[pair * kMatStride + tid] : 0.0;
+  __syncthreads();
+  double gm = 0.0;
+  if (tid < k)
+    gm = sd[tid] + (tid > 0 ? fabs(se[tid - 1]) : 0.0) + (tid < k - 1 ? fabs(se[tid]) : 0.0);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) gm = fmax(gm, __shfl_xor(gm, o, 64));
+  if (l == 0) sgm[tid >> 6] = gm;
+  __syncthreads();
+  gm = fmax(sgm[0], sgm[1]);
+  int bk = (int)ceil(log2(fmax(gm / m, 2.0)));
+  bk = bk < 1 ? 1 : (bk > nbucket - 1 ? nbucket - 1 : bk);
+  bk = __builtin_amdgcn_readfirstlane(bk);
+  const int nn = __builtin_amdgcn_readfirstlane(tabn[bk]);
+  const double* sn = tab + bk * 64;
+  const double* wn = sn + 32;
+  // unused node rows stay 0, so that their chains stay 0
+  for (int n = nn; n < kMaxNodes; ++n) { Us[n][tid] = 0.0; Gs[n][tid] = 0.0; }
+  // bottom-up pivots D- and U_i = e_(i-1) / D-_i (wave 1, one lane per node)
+  if (tid >= 64 && tid - 64 < nn) {
+    const int n = tid - 64;
+    const double sg = m * sn[n];
+    double Up = 0.0;
+    for (int i = k - 1; i >= 0; --i) {
+      const double Dm = sd[i] + sg - (i < k - 1 ? se[i] * Up : 0.0);
+      const double Ui = i > 0 ? se[i - 1] / Dm : 0.0;
+      Us[n][i] = Ui;
+      Gs[n][i] = Dm;
+      Up = Ui;
+    }
+  }
+  __syncthreads();
+  // top-down pivots D+ and the diagonal 1 / g_i of each inverse (wave 0)
+  if (tid < nn) {
+    const int n = tid;
+    const double sg = m * sn[n];
+    double Lp = 0.0;
+    for (int i = 0; i < k; ++i) {
+      const double Dp = sd[i] + sg - (i > 0 ? se[i - 1] * Lp : 0.0);
+      const double Li = i < k - 1 ? se[i] / Dp : 0.0;
+      Gs[n][i] = 1.0 / (Dp + Gs[n][i] - (sd[i] + sg));
+      Lp = Li;
+    }
+  }
+  __syncthreads();
+  const int c = tid;
+  if (c >= k) return;
+  // rows above a wave's first column hold zeros in every lane of the wave
+  const int rfirst = __builtin_amdgcn_readfirstlane(tid & ~63);
+  double* Bp = B + pair * (int64_t)k * k;
+  if (nn <= 16) mat_rational_rows<16>(k, c, rfirst, nn, m, wn, Us, Gs, Bp);
+  else if (nn <= 24) mat_rational_rows<24>(k, c, rfirst, nn, m, wn, Us, Gs, Bp);
+  else mat_rational_rows<kMaxNodes>(k, c, rfirst, nn, m, wn, Us, Gs, Bp);
+}
 
 // x_a = T x_b for every slab of the group: A operand T (rows = members),
 // B operand x_b (columns = slabs), K = the ensemble index.  Lane (i, half)
@@ -100,6 +182,7 @@ __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev
   }
   const float* xb0 = S.state + (int64_t)S.k * S.ns * ij;
   double* xa0 = ana + (int64_t)S.k * S.ns * ij;
+  const bool vec = (k & 3) == 0;
   for (int q0 = 0; q0 < nsl; q0 += 32) {
     const int qi = q0 + i;
     const int slab = qi < nsl ? sl[qi] : sl[0];
@@ -109,10 +192,16 @@ __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev
     for (int st = 0; st < KS; ++st) {
       if (st < ks) {
         float xf[8];
+        const int k0 = 16 * st + 8 * half;
+        if (vec && k0 + 8 <= k) {
+          // k % 4 == 0: the slab and this run of 8 members are 16-byte aligned
+          const float4 u0 = *reinterpret_cast<const float4*>(xp + k0);
+          const float4 u1 = *reinterpret_cast<const float4*>(xp + k0 + 4);
+          xf[0] = u0.x; xf[1] = u0.y; xf[2] = u0.z; xf[3] = u0.w;
+          xf[4] = u1.x; xf[5] = u1.y; xf[6] = u1.z; xf[7] = u1.w;
+        } else {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int kk = 16 * st + 8 * half + e;
-          xf[e] = kk < k ? xp[kk] : 0.0f;
+          for (int e = 0; e < 8; ++e) xf[e] = k0 + e < k ? xp[k0 + e] : 0.0f;
         }
         if (PREC == LETKF_PREC_FP32) {
 #pragma unroll
@@ -133,9 +222,17 @@ __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev
     if (qi < nsl) {
       double* op = xa0 + (int64_t)k * slab;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int mem = 32 * wv + 8 * (r >> 2) + 4 * half + (r & 3);
-        if (mem < k) op[mem] = (double)acc[r];
+      for (int r4 = 0; r4 < 4; ++r4) {
+        const int mem = 32 * wv + 8 * r4 + 4 * half;       // 4 consecutive members
+        if (vec && mem + 4 <= k) {
+          double2* o2 = reinterpret_cast<double2*>(op + mem);
+          o2[0] = make_double2((double)acc[4 * r4], (double)acc[4 * r4 + 1]);
+          o2[1] = make_double2((double)acc[4 * r4 + 2], (double)acc[4 * r4 + 3]);
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (mem + u < k) op[mem + u] = (double)acc[4 * r4 + u];
+        }
       }
     }
   }
@@ -194,17 +291,23 @@ bool transform_mat_supported(int k) { return k >= 64 && k <= 128; }
 
 void launch_transform_mat(hipStream_t st, int prec, const LocDev& L, const StateDev& S,
                           const VarsDev& V, const ChunkDev& C, const DiagDev& D, double* B,
-                          const double* bvec, int* status, const RatTab& R, double* ana) {
+                          const double* bvec, double* tau, double* dd, double* ee,
+                          const RatTab& R, double* ana) {
   const int64_t np = C.ngp * C.G;
   if (np <= 0) return;
   const int k = (int)S.k;
   const unsigned nb = (unsigned)np;
-  // rows per half: k <= 2 NR, NR even
-  const int nr = ((k + 1) / 2 + 1) & ~1;
-  if (nr <= 40) launch_transform_nr40(st, nb, k, C, B, bvec, status, R);
-  else if (nr <= 48) launch_transform_nr48(st, nb, k, C, B, bvec, status, R);
-  else if (nr <= 56) launch_transform_nr56(st, nb, k, C, B, bvec, status, R);
-  else launch_transform_nr64(st, nb, k, C, B, bvec, status, R);
+  // rows per half: k <= 2 NR
+  const int nr = (k + 1) / 2;
+  if (nr <= 40) launch_mat_tridiag_nr40(st, nb, k, C, B, tau, dd, ee);
+  else if (nr <= 48) launch_mat_tridiag_nr48(st, nb, k, C, B, tau, dd, ee);
+  else if (nr <= 56) launch_mat_tridiag_nr56(st, nb, k, C, B, tau, dd, ee);
+  else launch_mat_tridiag_nr64(st, nb, k, C, B, tau, dd, ee);
+  k_mat_rational<<<nb, 128, 0, st>>>(k, C, B, dd, ee, R.tab, R.n, R.nb);
+  if (nr <= 40) launch_mat_back_nr40(st, nb, k, C, B, bvec, tau);
+  else if (nr <= 48) launch_mat_back_nr48(st, nb, k, C, B, bvec, tau);
+  else if (nr <= 56) launch_mat_back_nr56(st, nb, k, C, B, bvec, tau);
+  else launch_mat_back_nr64(st, nb, k, C, B, bvec, tau);
   if (prec == LETKF_PREC_FP64)
     k_apply_mat_f64<<<nb, 256, 0, st>>>(L, S, V, C, D, B, ana);
   else if (prec == LETKF_PREC_FP32)

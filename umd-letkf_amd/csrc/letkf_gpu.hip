@@ -695,7 +695,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
   h->use_rat = nv > 0 && h->batched() && h->tune_rat && k >= 2;
   h->cap_rot = h->cap_it = 0;
   if (h->eigK && !h->use_rat) eigen_caps((int)k, &h->cap_rot, &h->cap_it);
-  const int64_t per_pair = 8 * k * k + 8 * k + 3 * 8 * (int64_t)h->eigK +
+  const int64_t per_pair = 8 * k * k + 8 * k + 3 * 8 * (int64_t)(use_mat ? 128 : h->eigK) +
                            (nv ? 8 * (int64_t)nv * h->eigK : 8 * (int64_t)h->eigK * h->eigK) +
                            16 * (int64_t)h->cap_rot + 16 * (int64_t)h->cap_it + 40;
   int64_t pairs = std::max<int64_t>(1, std::min<int64_t>(524288, (12ll << 30) / per_pair));
@@ -709,6 +709,11 @@ static int ensure_outputs(letkf_gpu_t* h) {
   HIPCHK(h, dalloc(&h->d_B, (size_t)(np * k * k)));
   HIPCHK(h, dalloc(&h->d_b, (size_t)(np * k)));
   HIPCHK(h, dalloc(&h->d_status, (size_t)np));
+  if (use_mat) {
+    HIPCHK(h, dalloc(&h->d_tau, (size_t)(np * 128)));
+    HIPCHK(h, dalloc(&h->d_dd, (size_t)(np * 128)));
+    HIPCHK(h, dalloc(&h->d_ee, (size_t)(np * 128)));
+  }
   if (h->eigK) {
     HIPCHK(h, dalloc(&h->d_tau, (size_t)(np * h->eigK)));
     HIPCHK(h, dalloc(&h->d_dd, (size_t)(np * h->eigK)));
@@ -862,7 +867,7 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     if (h->use_mat) {
       const RatTab R{h->d_rat, h->d_ratn, kRatBuckets};
       launch_transform_mat(st, h->batched() ? h->dims.precision : LETKF_PREC_FP64, h->loc, S,
-                           V, C, D, h->d_B, h->d_b, h->d_status, R, h->d_ana);
+                           V, C, D, h->d_B, h->d_b, h->d_tau, h->d_dd, h->d_ee, R, h->d_ana);
     } else if (h->eigK) {
       RatTab R;
       if (h->use_rat) { R.tab = h->d_rat; R.n = h->d_ratn; R.nb = kRatBuckets; }

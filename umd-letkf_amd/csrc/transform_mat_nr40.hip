@@ -1,12 +1,18 @@
-// transform_mat_nr40.hip -- k_transform_mat for rows of 40 per half
-// (2 x 40 >= k); one instantiation per file keeps the build parallel.
+// transform_mat_nr40.hip -- k_mat_tridiag and k_mat_back for rows of 40
+// per half (2 x 40 >= k); one instantiation per file keeps the build
+// parallel.
 #include "transform_mat.h"
 
 namespace lk {
 
-void launch_transform_nr40(hipStream_t st, unsigned nb, int k, const ChunkDev& C, double* B,
-                            const double* bvec, int* status, const RatTab& R) {
-  launch_transform_nr<40>(st, nb, k, C, B, bvec, status, R);
+void launch_mat_tridiag_nr40(hipStream_t st, unsigned nb, int k, const ChunkDev& C, double* B,
+                              double* tau, double* dd, double* ee) {
+  launch_mat_tridiag_nr<40>(st, nb, k, C, B, tau, dd, ee);
+}
+
+void launch_mat_back_nr40(hipStream_t st, unsigned nb, int k, const ChunkDev& C, double* B,
+                           const double* bvec, const double* tau) {
+  launch_mat_back_nr<40>(st, nb, k, C, B, bvec, tau);
 }
 
 }  // namespace lk
