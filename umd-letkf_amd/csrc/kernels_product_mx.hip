@@ -1,23 +1,23 @@
 // kernels_product_mx.hip -- Yb^T R^-1 Yb and Yb^T R^-1 d on low-precision
 // MFMA with FP32 accumulation (the batched mixed-precision path).
 //
-// One wave per (gridpoint, group) problem.  Problems are visited by
-// power-of-two buckets of their kept-observation count m, largest first
-// (a permutation sorted on the device), in gridpoint order inside a bucket,
-// and blocks are remapped so that each XCD walks a contiguous range.  The kept columns y_j (k values) are loaded
-// straight from the sorted hx array into MFMA operand registers, 16
-// observations per step, the next step's loads issued before the current
-// step's MFMAs.  Both MFMA operands are sqrt(rloc_j/err_j) y_j (the square
-// root written by the search's fill pass), so each value is rounded/split
-// once.  Row k of the padded operand carries the scaled d_j, so column k of
-// the result is Yb^T R^-1 d.  Only the upper-triangle 32x32 blocks are
-// computed.
+// One wave per (gridpoint, group) problem, launched in the order of a
+// device-sorted permutation (k_bucket_keys below: by default buckets of the
+// kept count m, largest first, gridpoint order inside).  The kept list of a
+// problem holds (column byte offset, sqrt(rloc_j/err_j)) pairs, read with the
+// padded operand columns hxp(kp, n) through buffer descriptors whose ranges
+// end at the list's zero-padded length and at the column array, so no
+// row or range test is needed.  Lists are loaded two 16-observation steps
+// ahead and columns one step ahead of their MFMAs.  Both MFMA operands are
+// sqrt(rloc_j/err_j) y_j, so each value is rounded or split once; row k of
+// the padded columns carries d_j, so column k of the result is Yb^T R^-1 d.
+// Only the upper-triangle 32x32 blocks are computed.
 //
 //   PREC_FP32    v_mfma_f32_32x32x2_f32   (8 MFMA per block per 16 obs)
 //   PREC_BF16    v_mfma_f32_32x32x16_bf16 (1 per block per 16 obs)
 //   PREC_BF16X3  split x = hi + lo in BF16, hi*hi + hi*lo + lo*hi (3 per block)
 //
-// The FP32 result is widened to FP64 for the eigensolve.
+// The FP32 result is widened to FP64 for the transform.
 #include <hip/hip_runtime.h>
 
 #include "letkf_internal.h"
@@ -38,11 +38,6 @@ __device__ __forceinline__ int obs_slot(int half, int e) {
   return PREC == LETKF_PREC_FP32 ? 2 * e + half : 8 * half + e;
 }
 
-// Operand registers of one 16-observation step, without row or range tests:
-// the kept list of a problem starts at a multiple of 8 entries and is
-// followed by at least 8 readable entries, and the padded columns hxp carry
-// the departure in row k and zeros above it.  Entries past m read column 0
-// with weight 0.
 // Operand registers of one 16-observation step.  The kept list of a problem
 // holds (column byte offset, weight) pairs from a multiple of 8 entries on,
 // zero-padded to the next multiple of 8; both are read through buffer
