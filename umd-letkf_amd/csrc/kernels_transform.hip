@@ -238,23 +238,27 @@ __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev
   }
 }
 
-// FP64 apply (the plain path): thread (i, half) accumulates member i of
-// every other slab, x_a(i) = sum_j T[i][j] x_b(j); x_b(j) is the same
-// address across the 128 threads of a slab (broadcast), T rows stay in L2
+// FP64 apply (the plain path) on FP64 MFMA (16x16x4): 16-member x 16-slab
+// output tiles; wave w owns the member tiles w and w + 4, keeps their T rows
+// as operand registers (lane l: member 16 t + l%16, K = 4 step + l/16) and
+// walks the slab tiles.  Output register r of lane l is member 4r + l/16 of
+// the tile, slab l%16.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_apply_mat_f64(LocDev L, StateDev S, VarsDev V,
                                                        ChunkDev C, DiagDev D,
                                                        const double* __restrict__ B,
                                                        double* __restrict__ ana) {
+  constexpr int KS = 32;             // K steps of 4, k <= 128
   __shared__ int sl[2048];
   __shared__ int snsl;
   const int64_t pair = blockIdx.x;
   if (!C.solve[pair]) return;
   const int k = (int)S.k;
-  const int tid = threadIdx.x, lane = tid & 63;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int64_t t_gp = pair / C.G;
   const int g = (int)(pair - t_gp * C.G);
   const int64_t ij = C.i0 + t_gp;
-  if (tid < 64) {
+  if (wv == 0) {
     const int mld = D.mld[ij];
     int nsl = 0;
     for (int s0 = 0; s0 < S.ns; s0 += 64) {
@@ -269,21 +273,42 @@ __global__ __launch_bounds__(256) void k_apply_mat_f64(LocDev L, StateDev S, Var
   }
   __syncthreads();
   const int nsl = snsl;
-  const int i = tid & 127, par = tid >> 7;
-  if (i >= k) return;
-  const double* Ti = B + pair * (int64_t)k * k + (int64_t)i * k;
+  const int ks = (k + 3) / 4;
+  const int i16 = lane & 15, kq = lane >> 4;
   const float* xb0 = S.state + (int64_t)S.k * S.ns * ij;
   double* xa0 = ana + (int64_t)S.k * S.ns * ij;
-  for (int q = par; q < nsl; q += 2) {
-    const float* xp = xb0 + (int64_t)k * sl[q];
-    double a0 = 0.0, a1 = 0.0;
-    int j = 0;
-    for (; j + 1 < k; j += 2) {
-      a0 = fma(Ti[j], (double)xp[j], a0);
-      a1 = fma(Ti[j + 1], (double)xp[j + 1], a1);
+  const double* Tb = B + pair * (int64_t)k * k;
+  for (int mt = wv; 16 * mt < k; mt += 4) {
+    const int row = 16 * mt + i16;
+    double af[KS];
+#pragma unroll
+    for (int st = 0; st < KS; ++st) {
+      const int kk = 4 * st + kq;
+      af[st] = (st < ks && row < k && kk < k) ? Tb[(int64_t)row * k + kk] : 0.0;
     }
-    if (j < k) a0 = fma(Ti[j], (double)xp[j], a0);
-    xa0[(int64_t)k * sl[q] + i] = a0 + a1;
+    for (int q0 = 0; q0 < nsl; q0 += 16) {
+      const int qi = q0 + i16;
+      const int slab = qi < nsl ? sl[qi] : sl[0];
+      const float* xp = xb0 + (int64_t)k * slab;
+      f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int st = 0; st < KS; ++st) {
+        if (st < ks) {
+          const int kk = 4 * st + kq;
+          const double xb = kk < k ? (double)xp[kk] : 0.0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[st], xb, acc, 0, 0, 0);
+        }
+      }
+      const int q = q0 + i16;
+      if (q < nsl) {
+        double* op = xa0 + (int64_t)k * sl[q];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int mem = 16 * mt + 4 * r + kq;
+          if (mem < k) op[mem] = acc[r];
+        }
+      }
+    }
   }
 }
 
