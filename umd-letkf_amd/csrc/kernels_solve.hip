@@ -14,6 +14,7 @@
 
 #include <algorithm>
 
+#include "eigen_common.h"
 #include "letkf_internal.h"
 
 namespace lk {
@@ -327,11 +328,7 @@ void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
 // for k > 64).  Every step of src/letkf_solver.F90:299-379 acts slab by slab,
 // so each slab is carried through all of them in turn.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
+__device__ __forceinline__ double wsum(double v) { return wave_sum_dpp(v); }
 
 __global__ __launch_bounds__(256) void k_finalize(LocDev L, StateDev S, VarsDev V,
                                                   ChunkDev C, DiagDev D, float rtps,

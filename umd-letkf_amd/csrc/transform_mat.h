@@ -41,30 +41,6 @@ constexpr int kMatStride = 128;   // d, e, tau entries per problem
 // ahead of their use (which would need more registers than a wave has)
 #define LK_FENCE(x) asm volatile("" : "+v"(x) :: "memory")
 
-// 64-bit DPP move of lane (row_shr / row_bcast) into lanes the row mask
-// enables, 0 elsewhere
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ double dpp_d(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, false);
-  return __hiloint2double(hi, lo);
-}
-
-// sum over the 64 lanes of a wave, returned wave-uniform: row_shr 1, 2, 4, 8
-// leave each 16-lane row's sum in its lane 15, row_bcast 15 and 31 carry
-// them on to lane 63
-__device__ __forceinline__ double wave_sum_dpp(double x) {
-  x += dpp_d<0x111, 0xf>(x);
-  x += dpp_d<0x112, 0xf>(x);
-  x += dpp_d<0x114, 0xf>(x);
-  x += dpp_d<0x118, 0xf>(x);
-  x += dpp_d<0x142, 0xa>(x);
-  x += dpp_d<0x143, 0xc>(x);
-  const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
-  return __hiloint2double(hi, lo);
-}
-
 template <int NR>
 __global__ __launch_bounds__(256) void k_mat_tridiag(int k, ChunkDev C, double* __restrict__ B,
                                                      double* __restrict__ tau,
@@ -121,12 +97,20 @@ __global__ __launch_bounds__(256) void k_mat_tridiag(int k, ChunkDev C, double* 
     const double t = refl ? (beta - alpha) / beta : 0.0;
     const double sc = refl ? 1.0 / (alpha - beta) : 0.0;
     // p = t A v over this half's rows (v: 1 at j+1, x sc below, 0 above)
+    // a wave whose rows or whose columns all lie at or above row j has
+    // nothing left to do: v and w vanish there
+    const bool live = r0 + NR - 1 > j && 64 * (wv & 1) + 63 > j;
     double pc = 0.0;
-    if (r0 + NR - 1 > j) {
+    if (live) {
 #pragma unroll
       for (int s = 0; s < NR; ++s) {
+        // branch-free: the LDS read is unconditional, the row tests only
+        // pick scalar factors (a conditional read became a branch and a
+        // full LDS wait per row)
         const int r = r0 + s;
-        const double vr = r == j1 ? 1.0 : (r > j1 ? sx[r] * sc : 0.0);
+        const double fr = r > j1 ? sc : 0.0;
+        const double ar = r == j1 ? 1.0 : 0.0;
+        const double vr = fma(sx[r], fr, ar);
         pc = fma(a[s], vr, pc);
         if ((s & 3) == 3) LK_FENCE(pc);
       }
@@ -146,7 +130,7 @@ __global__ __launch_bounds__(256) void k_mat_tridiag(int k, ChunkDev C, double* 
     }
     if (tid == 0) { stau[j] = t; se[j] = beta; }
     __syncthreads();
-    if (r0 + NR - 1 > j) {
+    if (live) {
 #pragma unroll
       for (int s = 0; s < NR; ++s) {
         const double2 q2 = svw[r0 + s];
