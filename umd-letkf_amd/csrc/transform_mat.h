@@ -36,6 +36,9 @@
 namespace lk {
 
 constexpr int kMatStride = 128;   // d, e, tau entries per problem
+// rows per fenced group in the unrolled row loops: the LDS reads of a group
+// are in flight together (more rows, more registers)
+constexpr int kFenceRows = 8;
 
 // keeps the scheduler from hoisting the LDS reads of an unrolled loop far
 // ahead of their use (which would need more registers than a wave has)
@@ -112,7 +115,7 @@ __global__ __launch_bounds__(256) void k_mat_tridiag(int k, ChunkDev C, double* 
         const double ar = r == j1 ? 1.0 : 0.0;
         const double vr = fma(sx[r], fr, ar);
         pc = fma(a[s], vr, pc);
-        if ((s & 3) == 3) LK_FENCE(pc);
+        if ((s & (kFenceRows - 1)) == kFenceRows - 1) LK_FENCE(pc);
       }
     }
     pp[h][c] = pc;
@@ -135,7 +138,7 @@ __global__ __launch_bounds__(256) void k_mat_tridiag(int k, ChunkDev C, double* 
       for (int s = 0; s < NR; ++s) {
         const double2 q2 = svw[r0 + s];
         a[s] = fma(-q2.x, wc, fma(-q2.y, vc, a[s]));
-        if ((s & 3) == 3) LK_FENCE(a[s]);
+        if ((s & (kFenceRows - 1)) == kFenceRows - 1) LK_FENCE(a[s]);
       }
     }
   }
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(256) void k_mat_back(int k, ChunkDev C, double* __r
 #pragma unroll
       for (int s = 0; s < NR; ++s) {
         pc = fma(a[s], svw[buf][r0 + s].x, pc);
-        if ((s & 3) == 3) LK_FENCE(pc);
+        if ((s & (kFenceRows - 1)) == kFenceRows - 1) LK_FENCE(pc);
       }
     }
     pp[h][c] = pc;
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(256) void k_mat_back(int k, ChunkDev C, double* __r
     for (int s = 0; s < NR; ++s) {
       const double2 q2 = svw[buf][r0 + s];
       a[s] = fma(-q2.x, wc, fma(-q2.y, vc, a[s]));
-      if ((s & 3) == 3) LK_FENCE(a[s]);
+      if ((s & (kFenceRows - 1)) == kFenceRows - 1) LK_FENCE(a[s]);
     }
   }
   // wbar = W (W b) / (k-1); T = W + 1 wbar^T, row major over B
@@ -222,7 +225,7 @@ __global__ __launch_bounds__(256) void k_mat_back(int k, ChunkDev C, double* __r
 #pragma unroll
   for (int s = 0; s < NR; ++s) {
     q = fma(a[s], sx[r0 + s], q);
-    if ((s & 3) == 3) LK_FENCE(q);
+    if ((s & (kFenceRows - 1)) == kFenceRows - 1) LK_FENCE(q);
   }
   pp[h][c] = q;
   __syncthreads();
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(256) void k_mat_back(int k, ChunkDev C, double* __r
 #pragma unroll
   for (int s = 0; s < NR; ++s) {
     q = fma(a[s], sx[r0 + s], q);
-    if ((s & 3) == 3) LK_FENCE(q);
+    if ((s & (kFenceRows - 1)) == kFenceRows - 1) LK_FENCE(q);
   }
   pp[h][c] = q;
   __syncthreads();
