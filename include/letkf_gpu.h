@@ -140,7 +140,10 @@ typedef struct letkf_gpu_timing {
  * count largest first, 3 buckets with a Morton walk inside; bucket_bits:
  * the bucket edges are the powers of two split into 2^bucket_bits parts
  * (0..2); tridiag_reg: 1 register-resident Householder reduction for
- * 32 <= k < 64, 0 the LDS kernel.  None of them changes the analysis. */
+ * 32 <= k < 64, 0 the LDS kernel; overlap: 1 searches the next chunk of
+ * gridpoints on a second stream while the current one is solved;
+ * chunk_log2: at most 2^chunk_log2 (gridpoint, group) problems per chunk
+ * (17..19).  None of them changes the analysis. */
 typedef struct letkf_gpu_tuning {
   int32_t order;
   int32_t bucket_bits;
@@ -148,6 +151,8 @@ typedef struct letkf_gpu_tuning {
   int32_t n_tried;     /* autotune: candidates timed                  */
   double best_ms;      /* autotune: analysis time of the chosen one   */
   double default_ms;   /* autotune: analysis time of the defaults     */
+  int32_t overlap;
+  int32_t chunk_log2;
 } letkf_gpu_tuning;
 
 typedef struct letkf_gpu_t letkf_gpu_t;
@@ -268,7 +273,8 @@ int letkf_gpu_get_timing(const letkf_gpu_t* h, letkf_gpu_timing* t);
 
 /* Autotuner (no counterpart in the reference): times the analysis of the
  * handle's current inputs under each launch choice (coordinate search:
- * order and bucket edges, then the tridiagonalization kernel), `repeats`
+ * order and bucket edges, the tridiagonalization kernel, the chunk overlap,
+ * then the chunk size), `repeats`
  * runs each (best taken, 1..16), keeps the fastest in the handle and
  * returns it.  Needs everything letkf_gpu_run needs; leaves the analysis
  * of the last timed run, identical under every choice. */

@@ -23,5 +23,6 @@ int main(void) {
   F(letkf_gpu_timing, order_ms);
   F(letkf_gpu_tuning, n_tried);
   F(letkf_gpu_tuning, default_ms);
+  F(letkf_gpu_tuning, chunk_log2);
   return 0;
 }

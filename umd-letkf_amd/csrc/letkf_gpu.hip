@@ -11,6 +11,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -85,6 +86,18 @@ struct letkf_gpu_t {
   int32_t* d_list_idx = nullptr;
   float* d_list_w = nullptr;
   int64_t list_cap = 0;
+  // second set of per-chunk search outputs and a second stream: the search
+  // of chunk c + 1 runs while chunk c is solved
+  hipStream_t stream2 = nullptr;
+  int64_t* d_cnt2 = nullptr;
+  int64_t* d_off2 = nullptr;
+  uint8_t* d_solve2 = nullptr;
+  int32_t* d_list_idx2 = nullptr;
+  float* d_list_w2 = nullptr;
+  int64_t list_cap2 = 0;
+  void* d_tmp2 = nullptr;
+  int64_t* h_tot2 = nullptr;   // pinned
+  int64_t alloc_chunk = 0;
   double *d_B = nullptr, *d_b = nullptr;
   // eigensolver work (per chunk pair)
   double *d_tau = nullptr, *d_dd = nullptr, *d_ee = nullptr;
@@ -115,6 +128,8 @@ struct letkf_gpu_t {
   int tune_bucket = 2;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
   int tune_bbits = 0;     // bucket edges: powers of two split in 2^bbits parts
   int tune_trireg = 1;    // register-resident tridiagonalization (32 <= k < 64)
+  int tune_overlap = 1;   // search of the next chunk concurrent with the solve
+  int tune_chunk_log2 = 19;   // at most 2^n problems per chunk (17..19)
   int tune_vec = 1;       // vector path when every group has few slabs
   int tune_rat = 1;       // batched vector path: rational A^(-1/2), no eigensolve
   int tune_mat = 1;       // batched, 64 <= k <= 128: explicit T applied on MFMA
@@ -197,15 +212,28 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   if (const char* e = getenv("LETKF_GPU_VEC")) h->tune_vec = atoi(e);
   if (const char* e = getenv("LETKF_GPU_RAT")) h->tune_rat = atoi(e);
   if (const char* e = getenv("LETKF_GPU_MAT")) h->tune_mat = atoi(e);
+  if (const char* e = getenv("LETKF_GPU_OVERLAP")) h->tune_overlap = atoi(e) != 0;
+  if (const char* e = getenv("LETKF_GPU_CHUNK_LOG2"))
+    h->tune_chunk_log2 = std::min(19, std::max(17, atoi(e)));
   hipGetDevice(&h->device);
+  // both streams at the default priority
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return fail(nullptr, LETKF_EHIP, "hipStreamCreate failed");
   }
+  if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess) {
+    hipStreamDestroy(h->stream);
+    delete h;
+    return fail(nullptr, LETKF_EHIP, "hipStreamCreate failed");
+  }
   if (hipHostMalloc((void**)&h->h_tot, 4 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&h->h_tot2, 4 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&h->h_ctr, sizeof(unsigned long long) * kCtrSlots * kCtrStride,
                     hipHostMallocDefault) != hipSuccess) {
+    if (h->h_tot) hipHostFree(h->h_tot);
+    if (h->h_tot2) hipHostFree(h->h_tot2);
     hipStreamDestroy(h->stream);
+    hipStreamDestroy(h->stream2);
     delete h;
     return fail(nullptr, LETKF_ENOMEM, "pinned allocation failed");
   }
@@ -216,6 +244,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
 int letkf_gpu_destroy(letkf_gpu_t* h) {
   if (!h) return LETKF_OK;
   hipStreamSynchronize(h->stream);
+  hipStreamSynchronize(h->stream2);
   dfree(h->d_state); dfree(h->d_mean); dfree(h->d_sprd); dfree(h->d_lat); dfree(h->d_lon);
   dfree(h->d_mask); dfree(h->d_gx); dfree(h->d_gy); dfree(h->d_gz);
   dfree(h->d_ox); dfree(h->d_oy); dfree(h->d_oz); dfree(h->d_ozdim); dfree(h->d_oerr);
@@ -229,13 +258,17 @@ int letkf_gpu_destroy(letkf_gpu_t* h) {
   dfree(h->d_lgloc); dfree(h->d_lghash);
   dfree(h->d_cnt); dfree(h->d_off); dfree(h->d_solve); dfree(h->d_list_idx);
   dfree(h->d_list_w); dfree(h->d_B); dfree(h->d_b); dfree(h->d_tmp);
+  dfree(h->d_cnt2); dfree(h->d_off2); dfree(h->d_solve2); dfree(h->d_list_idx2);
+  dfree(h->d_list_w2); dfree(h->d_tmp2);
   dfree(h->d_tau); dfree(h->d_dd); dfree(h->d_ee); dfree(h->d_rot); dfree(h->d_itr);
   dfree(h->d_nit); dfree(h->d_status); dfree(h->d_Z); dfree(h->d_U); dfree(h->d_nvec);
   dfree(h->d_rat); dfree(h->d_ratn);
   dfree(h->d_bkey); dfree(h->d_bkey2); dfree(h->d_bval); dfree(h->d_perm); dfree(h->d_stmp);
   if (h->h_tot) hipHostFree(h->h_tot);
+  if (h->h_tot2) hipHostFree(h->h_tot2);
   if (h->h_ctr) hipHostFree(h->h_ctr);
   hipStreamDestroy(h->stream);
+  hipStreamDestroy(h->stream2);
   delete h;
   return LETKF_OK;
 }
@@ -683,7 +716,8 @@ static int ensure_outputs(letkf_gpu_t* h) {
   const bool use_mat = !force_jacobi && h->tune_mat && transform_mat_supported((int)k);
   const int eigK = (force_jacobi || use_mat) ? 0 : eigen_pipeline_K((int)k);
   const int nv = (eigK && h->tune_vec) ? eigen_vec_nv(max_slabs_per_group(h)) : 0;
-  if (h->alloc_G == G && h->eigK == eigK && h->alloc_nv == nv && h->alloc_mat == (int)use_mat)
+  if (h->alloc_G == G && h->eigK == eigK && h->alloc_nv == nv && h->alloc_mat == (int)use_mat &&
+      h->alloc_chunk == h->tune_chunk_log2)
     return LETKF_OK;
   h->eigK = eigK;
   h->vec_nv = nv;
@@ -698,7 +732,8 @@ static int ensure_outputs(letkf_gpu_t* h) {
   const int64_t per_pair = 8 * k * k + 8 * k + 3 * 8 * (int64_t)(use_mat ? 128 : h->eigK) +
                            (nv ? 8 * (int64_t)nv * h->eigK : 8 * (int64_t)h->eigK * h->eigK) +
                            16 * (int64_t)h->cap_rot + 16 * (int64_t)h->cap_it + 40;
-  int64_t pairs = std::max<int64_t>(1, std::min<int64_t>(524288, (12ll << 30) / per_pair));
+  int64_t pairs = std::max<int64_t>(
+      1, std::min<int64_t>(1ll << h->tune_chunk_log2, (12ll << 30) / per_pair));
   int64_t cg = std::max<int64_t>(1, pairs / G);
   cg = std::min<int64_t>(cg, std::max<int64_t>(nij, 1));
   h->chunk_gp = cg;
@@ -706,6 +741,9 @@ static int ensure_outputs(letkf_gpu_t* h) {
   HIPCHK(h, dalloc(&h->d_cnt, (size_t)np));
   HIPCHK(h, dalloc(&h->d_off, (size_t)np));
   HIPCHK(h, dalloc(&h->d_solve, (size_t)np));
+  HIPCHK(h, dalloc(&h->d_cnt2, (size_t)np));
+  HIPCHK(h, dalloc(&h->d_off2, (size_t)np));
+  HIPCHK(h, dalloc(&h->d_solve2, (size_t)np));
   HIPCHK(h, dalloc(&h->d_B, (size_t)(np * k * k)));
   HIPCHK(h, dalloc(&h->d_b, (size_t)(np * k)));
   HIPCHK(h, dalloc(&h->d_status, (size_t)np));
@@ -734,6 +772,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
   HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, padded_counts(h->d_cnt), h->d_off,
                                              (int)np, h->stream));
   HIPCHK(h, dalloc((char**)&h->d_tmp, tb));
+  HIPCHK(h, dalloc((char**)&h->d_tmp2, tb));
   h->tmp_bytes = tb;
   if (h->batched()) {
     HIPCHK(h, dalloc(&h->d_bkey, (size_t)np));
@@ -765,6 +804,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
   h->alloc_G = G;
   h->alloc_nv = nv;
   h->alloc_mat = (int)use_mat;
+  h->alloc_chunk = h->tune_chunk_log2;
   return LETKF_OK;
 }
 
@@ -805,65 +845,108 @@ int letkf_gpu_run(letkf_gpu_t* h) {
   const size_t ctr_bytes = sizeof(unsigned long long) * kCtrSlots * kCtrStride;
   HIPCHK(h, hipMemsetAsync(h->d_ctr, 0, ctr_bytes, st));
 
-  std::vector<hipEvent_t> ev;
-  auto rec = [&]() -> int {
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return -1;
-    hipEventRecord(e, st);
-    ev.push_back(e);
-    return 0;
+  // Chunk pipeline.  With overlap, chunk c is searched on stream2 into
+  // buffer set c % 2 while chunk c - 1 is ordered, multiplied, transformed
+  // and finalized on the main stream; a set is reused only after the main
+  // stream has finished the chunk that last used it.
+  const bool overlap = h->tune_overlap != 0;
+  hipStream_t sst = overlap ? h->stream2 : st;
+  std::vector<hipEvent_t> evs;                  // all events, destroyed at the end
+  auto mk = [&](hipEvent_t* e) -> hipError_t {
+    hipError_t r = hipEventCreate(e);
+    if (r == hipSuccess) evs.push_back(*e);
+    return r;
   };
+  auto cleanup_events = [&]() { for (auto e : evs) hipEventDestroy(e); };
+#define RUNCHK(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      hipStreamSynchronize(st);                                                   \
+      hipStreamSynchronize(h->stream2);                                           \
+      cleanup_events();                                                           \
+      return fail(h, e_ == hipErrorOutOfMemory ? LETKF_ENOMEM : LETKF_EHIP,       \
+                  "%s: %s", #call, hipGetErrorString(e_));                        \
+    }                                                                             \
+  } while (0)
+  hipEvent_t slot_free[2] = {nullptr, nullptr};
+  hipEvent_t e_first = nullptr, e_last = nullptr;
+  // per chunk: search start / end (search stream), then order, product,
+  // transform, finalize boundaries (main stream)
+  std::vector<std::array<hipEvent_t, 7>> cev;
   letkf_gpu_timing T{};
-  rec();
-  for (int64_t i0 = 0; i0 < h->nij; i0 += h->chunk_gp) {
+  RUNCHK(mk(&e_first));
+  RUNCHK(hipEventRecord(e_first, st));
+  if (overlap) RUNCHK(hipStreamWaitEvent(sst, e_first, 0));   // counters cleared first
+  int chunk = 0;
+  for (int64_t i0 = 0; i0 < h->nij; i0 += h->chunk_gp, ++chunk) {
     const int64_t ngp = std::min(h->chunk_gp, h->nij - i0);
     const int64_t np = ngp * G;
-    ChunkDev C{i0, ngp, (int)G, h->d_cnt, h->d_off, h->d_solve, h->d_list_idx, h->d_list_w,
+    const int slot = overlap ? (chunk & 1) : 0;
+    int64_t* b_cnt = slot ? h->d_cnt2 : h->d_cnt;
+    int64_t* b_off = slot ? h->d_off2 : h->d_off;
+    uint8_t* b_solve = slot ? h->d_solve2 : h->d_solve;
+    int32_t*& b_list_idx = slot ? h->d_list_idx2 : h->d_list_idx;
+    float*& b_list_w = slot ? h->d_list_w2 : h->d_list_w;
+    int64_t& b_list_cap = slot ? h->list_cap2 : h->list_cap;
+    void* b_tmp = slot ? h->d_tmp2 : h->d_tmp;
+    int64_t* b_tot = slot ? h->h_tot2 : h->h_tot;
+    std::array<hipEvent_t, 7> ce{};
+    for (auto& e : ce) RUNCHK(mk(&e));
+    cev.push_back(ce);
+    if (overlap && slot_free[slot]) RUNCHK(hipStreamWaitEvent(sst, slot_free[slot], 0));
+    ChunkDev C{i0, ngp, (int)G, b_cnt, b_off, b_solve, b_list_idx, b_list_w,
                h->batched() ? 1 : 0, h->tune_xcd};
-    launch_search(st, false, h->loc, O, S, V, C, D);
-    HIPCHK(h, hipGetLastError());
+    RUNCHK(hipEventRecord(ce[0], sst));
+    launch_search(sst, false, h->loc, O, S, V, C, D);
+    RUNCHK(hipGetLastError());
     size_t tb = h->tmp_bytes;
-    HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_tmp, tb, padded_counts(h->d_cnt),
-                                               h->d_off, (int)np, st));
-    HIPCHK(h, hipMemcpyAsync(h->h_tot, h->d_off + (np - 1), 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipMemcpyAsync(h->h_tot + 1, h->d_cnt + (np - 1), 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipMemcpyAsync(h->h_ctr, h->d_ctr, ctr_bytes, hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipStreamSynchronize(st));
+    RUNCHK(hipcub::DeviceScan::ExclusiveSum(b_tmp, tb, padded_counts(b_cnt), b_off, (int)np,
+                                            sst));
+    RUNCHK(hipMemcpyAsync(b_tot, b_off + (np - 1), 8, hipMemcpyDeviceToHost, sst));
+    RUNCHK(hipMemcpyAsync(b_tot + 1, b_cnt + (np - 1), 8, hipMemcpyDeviceToHost, sst));
+    RUNCHK(hipMemcpyAsync(h->h_ctr, h->d_ctr, ctr_bytes, hipMemcpyDeviceToHost, sst));
+    RUNCHK(hipStreamSynchronize(sst));
     // list space: padded offsets, plus one alignment unit of slack for the
     // vector loads past the last list
-    const int64_t total = h->h_tot[0] + PadCount()(h->h_tot[1]) + kListAlign;
+    const int64_t total = b_tot[0] + PadCount()(b_tot[1]) + kListAlign;
     unsigned long long maxc = 0;
     for (int sl = 0; sl < kCtrSlots; ++sl)
       maxc = std::max(maxc, h->h_ctr[sl * kCtrStride + kCtrMax]);
-    if ((long long)maxc > (long long)h->dims.max_obs)
+    if ((long long)maxc > (long long)h->dims.max_obs) {
+      hipStreamSynchronize(st);
+      cleanup_events();
       return fail(h, LETKF_ECAPACITY, "a gridpoint has %llu observations in its search radius, "
                   "above max_obs = %lld", maxc, (long long)h->dims.max_obs);
-    if (total > h->list_cap) {
-      const int64_t cap = total + total / 4 + 1024;
-      HIPCHK(h, dalloc(&h->d_list_idx, (size_t)cap));
-      HIPCHK(h, dalloc(&h->d_list_w, (size_t)cap));
-      h->list_cap = cap;
-      C.list_idx = h->d_list_idx;
-      C.list_w = h->d_list_w;
     }
-    rec();
-    launch_search(st, true, h->loc, O, S, V, C, D);
-    HIPCHK(h, hipGetLastError());
-    rec();
+    if (total > b_list_cap) {
+      // this set's previous chunk is finished (the search stream waited for it)
+      const int64_t cap = total + total / 4 + 1024;
+      RUNCHK(dalloc(&b_list_idx, (size_t)cap));
+      RUNCHK(dalloc(&b_list_w, (size_t)cap));
+      b_list_cap = cap;
+      C.list_idx = b_list_idx;
+      C.list_w = b_list_w;
+    }
+    launch_search(sst, true, h->loc, O, S, V, C, D);
+    RUNCHK(hipGetLastError());
+    RUNCHK(hipEventRecord(ce[1], sst));
+    if (overlap) RUNCHK(hipStreamWaitEvent(st, ce[1], 0));
+    RUNCHK(hipEventRecord(ce[2], st));
     if (h->batched()) {
-      launch_bucket_keys(st, h->tune_bucket, h->tune_bbits, np, (int)G, i0, h->d_cnt, h->d_solve, h->d_lat,
-                         h->d_lon, h->d_bkey, h->d_bval);
+      launch_bucket_keys(st, h->tune_bucket, h->tune_bbits, np, (int)G, i0, b_cnt, b_solve,
+                         h->d_lat, h->d_lon, h->d_bkey, h->d_bval);
       size_t sb = h->stmp_bytes;
-      HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(h->d_stmp, sb, h->d_bkey, h->d_bkey2,
-                                                   h->d_bval, h->d_perm, (int)np, 0, 32, st));
-      rec();
+      RUNCHK(hipcub::DeviceRadixSort::SortPairs(h->d_stmp, sb, h->d_bkey, h->d_bkey2,
+                                                h->d_bval, h->d_perm, (int)np, 0, 32, st));
+      RUNCHK(hipEventRecord(ce[3], st));
       launch_product_mx(st, h->dims.precision, O, S, C, h->d_perm, h->d_B, h->d_b);
     } else {
-      rec();
+      RUNCHK(hipEventRecord(ce[3], st));
       launch_product_f64(st, O, S, C, h->d_B, h->d_b);
     }
-    HIPCHK(h, hipGetLastError());
-    rec();
+    RUNCHK(hipGetLastError());
+    RUNCHK(hipEventRecord(ce[4], st));
     if (h->use_mat) {
       const RatTab R{h->d_rat, h->d_ratn, kRatBuckets};
       launch_transform_mat(st, h->batched() ? h->dims.precision : LETKF_PREC_FP64, h->loc, S,
@@ -875,20 +958,45 @@ int letkf_gpu_run(letkf_gpu_t* h) {
                             h->d_ee, h->d_rot, h->cap_rot, h->d_itr, h->cap_it, h->d_nit,
                             h->d_status, h->d_Z, h->vec_nv, h->d_U, h->d_nvec, R, h->d_ana,
                             h->tune_trireg != 0);
-      HIPCHK(h, hipGetLastError());
+      RUNCHK(hipGetLastError());
       // problems the QL could not finish go to the Jacobi solver
       launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_status, h->d_ana);
     } else {
       launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, nullptr, h->d_ana);
     }
-    HIPCHK(h, hipGetLastError());
-    rec();
+    RUNCHK(hipGetLastError());
+    RUNCHK(hipEventRecord(ce[5], st));
     launch_finalize(st, h->loc, S, V, C, D, h->rtps, h->rtpp, h->mul, h->d_ana, h->d_ana,
                     h->d_amean, h->d_asprd);
-    HIPCHK(h, hipGetLastError());
-    rec();
-    (void)k;
+    RUNCHK(hipGetLastError());
+    RUNCHK(hipEventRecord(ce[6], st));
+    slot_free[slot] = ce[6];
   }
+  RUNCHK(mk(&e_last));
+  RUNCHK(hipEventRecord(e_last, st));
+  RUNCHK(hipStreamSynchronize(st));
+  RUNCHK(hipStreamSynchronize(h->stream2));
+  for (const auto& ce : cev) {
+    float a = 0, o = 0, p = 0, e = 0, f = 0;
+    hipEventElapsedTime(&a, ce[0], ce[1]);
+    hipEventElapsedTime(&o, ce[2], ce[3]);
+    hipEventElapsedTime(&p, ce[3], ce[4]);
+    hipEventElapsedTime(&e, ce[4], ce[5]);
+    hipEventElapsedTime(&f, ce[5], ce[6]);
+    T.search_ms += a;
+    T.order_ms += o;
+    T.product_ms += p;
+    T.eigen_ms += e;
+    T.finalize_ms += f;
+    T.n_launch += 1;
+  }
+  {
+    float tt = 0;
+    hipEventElapsedTime(&tt, e_first, e_last);
+    T.total_ms = tt;
+  }
+  cleanup_events();
+#undef RUNCHK
   {
     HIPCHK(h, hipMemcpyAsync(h->h_ctr, h->d_ctr, ctr_bytes, hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipStreamSynchronize(st));
@@ -898,29 +1006,6 @@ int letkf_gpu_run(letkf_gpu_t* h) {
       T.n_pairs += (int64_t)h->h_ctr[sl * kCtrStride + kCtrPairs];
     }
   }
-  // events: start, then per chunk [count+scan, fill, order, product, eigen,
-  // finalize]
-  for (size_t c = 0; c + 6 < ev.size(); c += 6) {
-    float a, b, o, p, e, f;
-    hipEventElapsedTime(&a, ev[c], ev[c + 1]);
-    hipEventElapsedTime(&b, ev[c + 1], ev[c + 2]);
-    hipEventElapsedTime(&o, ev[c + 2], ev[c + 3]);
-    hipEventElapsedTime(&p, ev[c + 3], ev[c + 4]);
-    hipEventElapsedTime(&e, ev[c + 4], ev[c + 5]);
-    hipEventElapsedTime(&f, ev[c + 5], ev[c + 6]);
-    T.search_ms += a + b;
-    T.order_ms += o;
-    T.product_ms += p;
-    T.eigen_ms += e;
-    T.finalize_ms += f;
-    T.n_launch += 1;
-  }
-  if (ev.size() >= 2) {
-    float tt;
-    hipEventElapsedTime(&tt, ev.front(), ev.back());
-    T.total_ms = tt;
-  }
-  for (auto e : ev) hipEventDestroy(e);
   h->timing = T;
   h->have_run = true;
   return LETKF_OK;
@@ -1083,7 +1168,8 @@ int letkf_gpu_get_timing(const letkf_gpu_t* h, letkf_gpu_timing* t) {
 
 int letkf_gpu_get_tuning(const letkf_gpu_t* h, letkf_gpu_tuning* out) {
   if (!h || !out) return LETKF_EINVAL;
-  *out = letkf_gpu_tuning{h->tune_bucket, h->tune_bbits, h->tune_trireg, 0, 0.0, 0.0};
+  *out = letkf_gpu_tuning{h->tune_bucket, h->tune_bbits, h->tune_trireg, 0, 0.0, 0.0,
+                          h->tune_overlap, h->tune_chunk_log2};
   return LETKF_OK;
 }
 
@@ -1095,9 +1181,15 @@ int letkf_gpu_set_tuning(letkf_gpu_t* h, const letkf_gpu_tuning* t) {
     return fail(h, LETKF_EINVAL, "tuning bucket_bits %d outside 0..2", t->bucket_bits);
   if (t->tridiag_reg != 0 && t->tridiag_reg != 1)
     return fail(h, LETKF_EINVAL, "tuning tridiag_reg %d not 0 or 1", t->tridiag_reg);
+  if (t->overlap != 0 && t->overlap != 1)
+    return fail(h, LETKF_EINVAL, "tuning overlap %d not 0 or 1", t->overlap);
+  if (t->chunk_log2 < 17 || t->chunk_log2 > 19)
+    return fail(h, LETKF_EINVAL, "tuning chunk_log2 %d outside 17..19", t->chunk_log2);
   h->tune_bucket = t->order;
   h->tune_bbits = t->bucket_bits;
   h->tune_trireg = t->tridiag_reg;
+  h->tune_overlap = t->overlap;
+  h->tune_chunk_log2 = t->chunk_log2;
   return LETKF_OK;
 }
 
@@ -1154,6 +1246,20 @@ int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out) {
     letkf_gpu_tuning t = best;
     t.tridiag_reg = 1 - t.tridiag_reg;
     if ((rc = consider(t))) return rc;
+  }
+  // overlap and chunk size matter only when the smallest chunk size would
+  // split the analysis
+  if (h->nij * h->G > (1ll << 17)) {
+    letkf_gpu_tuning t = best;
+    t.overlap = 1 - t.overlap;
+    if ((rc = consider(t))) return rc;
+    const letkf_gpu_tuning base = best;
+    for (int cl = 17; cl <= 19; ++cl) {
+      if (cl == base.chunk_log2) continue;
+      letkf_gpu_tuning u = base;
+      u.chunk_log2 = cl;
+      if ((rc = consider(u))) return rc;
+    }
   }
   // leave the chosen settings and their analysis in the handle
   rc = letkf_gpu_set_tuning(h, &best);
