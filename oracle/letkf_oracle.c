@@ -553,7 +553,14 @@ typedef struct {
   int64_t cap_m;
   double* bkg; double* ana; double* trans;
   int* gid; int* slab_group;
+  double t_search, t_core, t_trans;   /* phase thread-seconds */
 } orc_scratch;
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
 
 static int scratch_init(orc_scratch* s, const orc_problem* p, int maxgroups) {
   memset(s, 0, sizeof(*s));
@@ -623,6 +630,7 @@ static int analyse_point(const orc_ctx* c, orc_scratch* s, int64_t ij,
     }
     return 0;
   }
+  double tp = now_s();
   float lat_ij = p->lat[ij];
   float maxhz = loc_maxhz(&c->loc, lat_ij);                  /* :210-212 */
   float rs = maxhz * ORC_STDEV2MAX;
@@ -631,6 +639,11 @@ static int analyse_point(const orc_ctx* c, orc_scratch* s, int64_t ij,
   orc_xyz(p->lat[ij], p->lon[ij], g);
   int64_t n = search(c, g, orc_thr2(rs), s->cidx, s->cdist);  /* :217-220 */
   if (out->obscount) out->obscount[slot] = n;
+  {
+    const double t = now_s();
+    s->t_search += t - tp;
+    tp = t;
+  }
 
   int ngr = loc_groups(&c->loc, p, ij, s->gid, s->slab_group);  /* :232 */
   for (int64_t i = 0; i < k * ns; ++i) { s->bkg[i] = (double)st[i]; s->ana[i] = s->bkg[i]; }
@@ -638,8 +651,12 @@ static int analyse_point(const orc_ctx* c, orc_scratch* s, int64_t ij,
   const float* bsprd = p->sprd + ns * ij;
 
   for (int lg = 0; lg < ngr; ++lg) {                          /* :241-292 */
+    double tq = now_s();
+    s->t_trans += tq - tp;
     int64_t m = localize_group(c, lat_ij, s->gid[lg], n, s->cidx, s->cdist,
                                s->kidx, s->krloc);
+    tp = now_s();
+    s->t_search += tp - tq;
     double sumr = 0.0;
     uint64_t h = 0;
     for (int64_t i = 0; i < m; ++i) { sumr += s->krloc[i]; h += orc_mix64((uint64_t)s->kidx[i]); }
@@ -660,6 +677,9 @@ static int analyse_point(const orc_ctx* c, orc_scratch* s, int64_t ij,
     }
     if (orc_core_solve((int)k, (int)m, s->hdxb, s->rdiag, s->rloc, s->dep, 1.0,
                        s->trans, NULL)) return -1;
+    tq = now_s();
+    s->t_core += tq - tp;
+    tp = tq;
     for (int64_t sl = 0; sl < ns; ++sl) {                      /* :285-289 */
       if (s->slab_group[sl] != lg) continue;
       const double* xb = s->bkg + k * sl;
@@ -743,6 +763,7 @@ static int analyse_point(const orc_ctx* c, orc_scratch* s, int64_t ij,
     osprd[sl] = sqrt(ss / (double)(k - 1));
     for (int64_t j = 0; j < k; ++j) ost[j + k * sl] = x[j] + omean[sl];
   }
+  s->t_trans += now_s() - tp;
   return 0;
 }
 
@@ -753,6 +774,7 @@ typedef struct {
   int tid, nthreads;
   const orc_output* out;
   int status;
+  double t_search, t_core, t_trans;
 } worker_arg;
 
 static void* worker(void* argp) {
@@ -763,15 +785,13 @@ static void* worker(void* argp) {
   for (int64_t i = a->tid; i < a->npts; i += a->nthreads) {
     if (analyse_point(a->c, &s, a->pts[i], i, a->out)) { a->status = -1; break; }
   }
+  a->t_search = s.t_search;
+  a->t_core = s.t_core;
+  a->t_trans = s.t_trans;
   scratch_free(&s);
   return NULL;
 }
 
-static double now_s(void) {
-  struct timespec ts;
-  clock_gettime(CLOCK_MONOTONIC, &ts);
-  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
-}
 
 int orc_analysis(const orc_problem* p, const int64_t* pts, int64_t npts,
                  int nthreads, const orc_output* out) {
@@ -785,7 +805,7 @@ int orc_analysis(const orc_problem* p, const int64_t* pts, int64_t npts,
   worker_arg args[256];
   int status = 0;
   for (int t = 0; t < nthreads; ++t) {
-    args[t] = (worker_arg){&c, pts, npts, t, nthreads, out, 0};
+    args[t] = (worker_arg){&c, pts, npts, t, nthreads, out, 0, 0.0, 0.0, 0.0};
     if (nthreads == 1) worker(&args[t]);
     else pthread_create(&th[t], NULL, worker, &args[t]);
   }
@@ -796,6 +816,12 @@ int orc_analysis(const orc_problem* p, const int64_t* pts, int64_t npts,
   if (out->seconds) {
     out->seconds[0] = t1 - t0;
     out->seconds[1] = now_s() - t1;
+    out->seconds[2] = out->seconds[3] = out->seconds[4] = 0.0;
+    for (int t = 0; t < nthreads; ++t) {
+      out->seconds[2] += args[t].t_search;
+      out->seconds[3] += args[t].t_core;
+      out->seconds[4] += args[t].t_trans;
+    }
   }
   ctx_free(&c);
   return status;

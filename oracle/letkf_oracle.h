@@ -56,7 +56,12 @@ typedef struct orc_output {
   int64_t* lg_m;        /* (maxgroups, npts) */
   double* lg_obsloc;    /* (maxgroups, npts) */
   uint64_t* lg_hash;    /* (maxgroups, npts) */
-  double* seconds;      /* optional [2]: search-structure set-up, gridpoint loop */
+  double* seconds;      /* optional [5]: search-structure set-up, gridpoint loop
+                           (wall), then thread-seconds summed over the
+                           workers in the reference's timer phases:
+                           obs_search (search, localization, filter),
+                           letkf_core (core solve), letkf_trans (apply and
+                           post-processing) */
 } orc_output;
 
 /* deterministic elementary functions (shared definition with the GPU) */

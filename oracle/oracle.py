@@ -219,7 +219,7 @@ def analysis(prob, points=None, nthreads=None):
     o.lg_m = _p(out["lg_m"], C.c_int64)
     o.lg_obsloc = _p(out["lg_obsloc"], C.c_double)
     o.lg_hash = _p(out["lg_hash"], C.c_uint64)
-    secs = np.zeros(2)
+    secs = np.zeros(5)
     o.seconds = _p(secs, C.c_double)
     nt = nthreads or min(os.cpu_count() or 1, 16)
     rc = lib().orc_analysis(C.byref(b.s), _p(pts, C.c_int64), n, nt, C.byref(o))
@@ -227,6 +227,8 @@ def analysis(prob, points=None, nthreads=None):
         raise RuntimeError("oracle analysis failed")
     out["points"] = pts
     out["setup_s"], out["loop_s"] = float(secs[0]), float(secs[1])
+    # thread-seconds in the reference's timer phases
+    out["obs_search_s"], out["letkf_core_s"], out["letkf_trans_s"] = (float(x) for x in secs[2:])
     return out
 
 
