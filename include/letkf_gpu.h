@@ -18,8 +18,9 @@
  *  - The caller owns every host buffer.  The set_* calls copy host data to
  *    device memory owned by the handle; the get_* calls copy results back.
  *    Nothing the caller passes is kept or freed.
- *  - Calls are synchronous.  A handle owns one HIP stream (default
- *    priority) and is not thread safe; use one handle per host thread.
+ *  - Calls are synchronous.  A handle owns two HIP streams (default
+ *    priority: the solve, and the search of the next chunk of gridpoints)
+ *    and is not thread safe; use one handle per host thread.
  *  - Fortran LOGICAL arrays are passed as int32 (non-zero = .TRUE.).
  *  - Gridpoint, slab and observation indices are 0-based at this ABI.
  */

@@ -86,7 +86,9 @@ def test_vector_and_full_eigen_paths(case, vec, monkeypatch):
     _check(p)
 
 
-def test_k_above_pipeline_uses_jacobi():
+def test_k70_explicit_transform():
+    """k = 70 is past the tridiagonal-QL pipeline (k < 64): the plain path
+    forms the transform matrix explicitly and applies it in FP64."""
     p = synth.random_problem(seed=77, k=70, ns=1, nobs=1500)
     _check(p)
 
