@@ -276,7 +276,8 @@ int letkf_gpu_get_timing(const letkf_gpu_t* h, letkf_gpu_timing* t);
  * handle's current inputs under each launch choice (coordinate search:
  * order and bucket edges, the tridiagonalization kernel, the chunk overlap,
  * then the chunk size), `repeats`
- * runs each (best taken, 1..16), keeps the fastest in the handle and
+ * runs each (best taken, 1..16), keeps the fastest (a candidate must beat
+ * the current choice by 0.5%) in the handle and
  * returns it.  Needs everything letkf_gpu_run needs; leaves the analysis
  * of the last timed run, identical under every choice. */
 int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out);

@@ -128,7 +128,7 @@ struct letkf_gpu_t {
   int tune_bucket = 2;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
   int tune_bbits = 0;     // bucket edges: powers of two split in 2^bbits parts
   int tune_trireg = 1;    // register-resident tridiagonalization (32 <= k < 64)
-  int tune_overlap = 1;   // search of the next chunk concurrent with the solve
+  int tune_overlap = 0;   // search of the next chunk concurrent with the solve
   int tune_chunk_log2 = 19;   // at most 2^n problems per chunk (17..19)
   int tune_vec = 1;       // vector path when every group has few slabs
   int tune_rat = 1;       // batched vector path: rational A^(-1/2), no eigensolve
@@ -1222,11 +1222,13 @@ int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out) {
   rc = time_of(best, &best_ms);
   if (rc) return rc;
   const double default_ms = best_ms;
+  // a candidate replaces the current choice only if it is at least 0.5%
+  // faster (timing noise does not move the choice)
   auto consider = [&](letkf_gpu_tuning t) -> int {
     double ms = 0.0;
     const int r = time_of(t, &ms);
     if (r) return r;
-    if (ms < best_ms) { best = t; best_ms = ms; }
+    if (ms < 0.995 * best_ms) { best = t; best_ms = ms; }
     return LETKF_OK;
   };
   // order and bucket edges (the batched product only), then the
