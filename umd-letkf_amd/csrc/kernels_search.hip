@@ -82,6 +82,15 @@ __device__ __forceinline__ bool group_has_slabs(const LocDev& L, int g, int mld)
 //   kFillDiag    one group: lists written at the bound offsets, then the kept
 //                count, the solve flag and the diagnostics
 constexpr int kCountExact = 0, kFill = 1, kCountBound = 2, kFillDiag = 3;
+constexpr int kQueue = 128;   // per-wave candidate ring (two batches)
+
+// orders this wave's LDS stores and loads on either side (the queue is
+// private to one wave; the block's waves retire independently, so no
+// block barrier)
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
 
 template <int PASS>
 __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
@@ -92,6 +101,8 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
   __shared__ double s_sum[4][kMaxGroups];
   __shared__ uint64_t s_hash[4][kMaxGroups];
   __shared__ int64_t s_lim[4][kMaxGroups];       // fill: end of each list's space
+  __shared__ int64_t s_qs[4][kQueue];             // queued candidates: obs index
+  __shared__ double s_qc[4][kQueue];              //   and squared chord
   const int wid = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t t = (int64_t)blockIdx.x * 4 + wid;
@@ -188,6 +199,73 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
   int64_t nin = 0, ncand = 0;
   double lsum = 0.0;          // kFillDiag: this lane's weights and hashes
   uint64_t lhash = 0;
+
+  // Candidates inside the cap go through a per-wave queue in LDS (a ring of
+  // 128 entries, in scan order) and are localized 64 at a time, so that the
+  // distance, weight, hash and list work runs on full waves; lists are
+  // written in the same scan order as without the queue.
+  int qh = 0, qn = 0;          // ring head and length (wave-uniform)
+  auto localize = [&](int cnt) {
+    wave_lds_fence();
+    const bool inside = lane < cnt;
+    int64_t s = 0;
+    double c2 = 9.0;
+    if (inside) {
+      const int e = (qh + lane) & (kQueue - 1);
+      s = s_qs[wid][e];
+      c2 = s_qc[wid][e];
+    }
+    wave_lds_fence();
+    float d = 0.0f, hz = 0.0f, zd = 0.0f, twp = 1.0f, tws = 1.0f;
+    uint8_t sf = 0;
+    uint64_t mix = 0;
+    if (inside) {
+      d = dm::dist(c2);
+      if (L.cls != LETKF_LOC_OCEAN) hz = dm::gc(d, Lh);
+      if (L.cls == LETKF_LOC_OCEAN) { sf = O.surf[s]; twp = O.tw_prof[s]; tws = O.tw_sat[s]; }
+      if (L.cls == LETKF_LOC_VGAUSS) zd = O.zdim[s];
+      if (STATS) mix = dm::mix64((uint64_t)O.oidx[s]);
+    }
+    for (int g = 0; g < G; ++g) {
+      float w = 0.0f;
+      if (inside) w = loc_weight(L, g, hz, d, Lp, Ls, sf, twp, tws, zd, V.lev_depth);
+      const bool keep = inside && (w > 1.0e-4f);   // src/letkf_solver.F90:251
+      const uint64_t kmask = __ballot(keep);
+      if (kmask == 0) continue;
+      if (PASS == kFillDiag) {
+        // one group: per-lane partial sums, reduced once per gridpoint
+        lsum += keep ? (double)w : 0.0;
+        lhash += keep ? mix : 0ull;
+      } else if (STATS) {
+        const double sr = wave_sum_f64(keep ? (double)w : 0.0);
+        const uint64_t hs = wave_sum_u64(keep ? mix : 0ull);
+        if (lane == 0) {
+          if (!FILL) s_cnt[wid][g] += __popcll(kmask);
+          s_sum[wid][g] += sr;
+          s_hash[wid][g] += hs;
+        }
+      }
+      if (FILL) {
+        // writes never pass the end of this pair's list space
+        int64_t cur = 0, lim = 0;
+        if (lane == 0) { cur = s_cnt[wid][g]; lim = s_lim[wid][g]; }
+        cur = __shfl(cur, 0, 64);
+        lim = __shfl(lim, 0, 64);
+        if (keep) {
+          const int64_t pos = cur + lane_rank(kmask);
+          if (pos < lim) {
+            // mixed products: byte offset of the padded operand column
+            C.list_idx[pos] = C.list_mode ? (int32_t)(s * O.kp * 4) : (int32_t)s;
+            C.list_w[pos] = C.list_mode ? sqrtf(w / O.err[s]) : w;
+          }
+        }
+        if (lane == 0) s_cnt[wid][g] = cur + __popcll(kmask);
+      }
+    }
+    qh = (qh + cnt) & (kQueue - 1);
+    qn -= cnt;
+  };
+
   for (int b = b0; b <= b1; ++b) {
     for (int r = 0; r < nr; ++r) {
       const int64_t sb = O.cell_start[b * kLonCells + rc0[r]];
@@ -203,55 +281,18 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
         if (imask == 0) continue;
         nin += __popcll(imask);
         if (PASS == kCountBound) continue;
-        float d = 0.0f, hz = 0.0f, zd = 0.0f, twp = 1.0f, tws = 1.0f;
-        uint8_t sf = 0;
-        uint64_t mix = 0;
+        // qn < 64 here, so the ring (128) has room for the whole batch
         if (inside) {
-          d = dm::dist(c2);
-          if (L.cls != LETKF_LOC_OCEAN) hz = dm::gc(d, Lh);
-          if (L.cls == LETKF_LOC_OCEAN) { sf = O.surf[s]; twp = O.tw_prof[s]; tws = O.tw_sat[s]; }
-          if (L.cls == LETKF_LOC_VGAUSS) zd = O.zdim[s];
-          if (STATS) mix = dm::mix64((uint64_t)O.oidx[s]);
+          const int e = (qh + qn + lane_rank(imask)) & (kQueue - 1);
+          s_qs[wid][e] = s;
+          s_qc[wid][e] = c2;
         }
-        for (int g = 0; g < G; ++g) {
-          float w = 0.0f;
-          if (inside) w = loc_weight(L, g, hz, d, Lp, Ls, sf, twp, tws, zd, V.lev_depth);
-          const bool keep = inside && (w > 1.0e-4f);   // src/letkf_solver.F90:251
-          const uint64_t kmask = __ballot(keep);
-          if (kmask == 0) continue;
-          if (PASS == kFillDiag) {
-            // one group: per-lane partial sums, reduced once per gridpoint
-            lsum += keep ? (double)w : 0.0;
-            lhash += keep ? mix : 0ull;
-          } else if (STATS) {
-            const double sr = wave_sum_f64(keep ? (double)w : 0.0);
-            const uint64_t hs = wave_sum_u64(keep ? mix : 0ull);
-            if (lane == 0) {
-              if (!FILL) s_cnt[wid][g] += __popcll(kmask);
-              s_sum[wid][g] += sr;
-              s_hash[wid][g] += hs;
-            }
-          }
-          if (FILL) {
-            // writes never pass the end of this pair's list space
-            int64_t cur = 0, lim = 0;
-            if (lane == 0) { cur = s_cnt[wid][g]; lim = s_lim[wid][g]; }
-            cur = __shfl(cur, 0, 64);
-            lim = __shfl(lim, 0, 64);
-            if (keep) {
-              const int64_t pos = cur + lane_rank(kmask);
-              if (pos < lim) {
-                // mixed products: byte offset of the padded operand column
-                C.list_idx[pos] = C.list_mode ? (int32_t)(s * O.kp * 4) : (int32_t)s;
-                C.list_w[pos] = C.list_mode ? sqrtf(w / O.err[s]) : w;
-              }
-            }
-            if (lane == 0) s_cnt[wid][g] = cur + __popcll(kmask);
-          }
-        }
+        qn += __popcll(imask);
+        if (qn >= 64) localize(64);
       }
     }
   }
+  if (PASS != kCountBound && qn > 0) localize(qn);
 
   if (PASS == kFillDiag) {
     const double sr = wave_sum_f64(lsum);
