@@ -103,6 +103,7 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
   __shared__ int64_t s_lim[4][kMaxGroups];       // fill: end of each list's space
   __shared__ int64_t s_qs[4][kQueue];             // queued candidates: obs index
   __shared__ double s_qc[4][kQueue];              //   and squared chord
+  __shared__ double s_dl[4][64];                  // longitude half-width per band
   const int wid = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t t = (int64_t)blockIdx.x * 4 + wid;
@@ -165,36 +166,42 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
 
   const double gx = S.gx[ij], gy = S.gy[ij], gz = S.gz[ij];
 
-  // conservative cell cover of the cap
+  // conservative cell cover of the cap: bands b0..b1 and, in each band,
+  // the longitude cells the cap can reach within that band
   const double theta = (double)rs / dm::kRE * (1.0 + 1e-6) + 1e-9;
   const double lat0 = (double)lat_ij;
   double lon0 = (double)S.lon[ij];
   lon0 = lon0 - 360.0 * floor(lon0 / 360.0);
   int b0 = 0, b1 = kBands - 1;
-  bool full_lon = true;
-  double dl = 180.0;
+  bool all_lon = true;          // the cap reaches a pole: whole bands
+  double thd = 180.0;
   if (theta < 3.14159) {
-    const double thd = theta * kRad2Deg;
+    thd = theta * kRad2Deg;
     b0 = (int)floor(lat0 - thd + 90.0);
     b1 = (int)floor(lat0 + thd + 90.0);
     b0 = b0 < 0 ? 0 : (b0 >= kBands ? kBands - 1 : b0);
     b1 = b1 < 0 ? 0 : (b1 >= kBands ? kBands - 1 : b1);
-    if (lat0 + thd < 90.0 && lat0 - thd > -90.0) {
-      const double sn = sin(theta), cp = cos(lat0 / kRad2Deg);
-      if (sn < cp) {
-        dl = asin(sn / cp) * kRad2Deg * (1.0 + 1e-9) + 1e-6;
-        full_lon = dl >= 180.0;
-      }
-    }
+    all_lon = !(lat0 + thd < 90.0 && lat0 - thd > -90.0);
   }
-  int nr = 1;
-  int rc0[2] = {0, 0}, rc1[2] = {kLonCells - 1, kLonCells - 1};
-  if (!full_lon) {
-    int c0 = (int)floor(lon0 - dl), c1 = (int)floor(lon0 + dl);
-    if (c0 < 0) { nr = 2; rc0[0] = c0 + kLonCells; rc1[0] = kLonCells - 1; rc0[1] = 0; rc1[1] = c1; }
-    else if (c1 >= kLonCells) { nr = 2; rc0[0] = c0; rc1[0] = kLonCells - 1; rc0[1] = 0; rc1[1] = c1 - kLonCells; }
-    else { rc0[0] = c0; rc1[0] = c1; }
-  }
+  const double sh = sin(0.5 * theta), hav_cap = sh * sh;
+  const double cos_lat0 = cos(lat0 / kRad2Deg);
+  // longitude half-width (degrees) that bounds the cap within band b, from
+  //   hav(dlon) = (hav(cap) - hav(dlat)) / (cos lat0 cos lat),
+  // with dlat at its smallest over the band and |lat| at its largest over
+  // the band's part of the cap's latitude range; < 0: the band misses the cap
+  auto band_halfwidth = [&](int b) -> double {
+    if (all_lon) return 180.0;
+    const double la = (double)b - 90.0, lb = la + 1.0;
+    const double dmin = lat0 < la ? la - lat0 : (lat0 > lb ? lat0 - lb : 0.0);
+    const double sd = sin(0.5 * dmin / kRad2Deg);
+    const double num = hav_cap - sd * sd;
+    if (num <= 0.0) return -1.0;
+    const double lo = fmax(la, lat0 - thd), hi = fmin(lb, lat0 + thd);
+    const double amax = lo <= hi ? fmax(fabs(lo), fabs(hi)) : fmax(fabs(la), fabs(lb));
+    const double q = num / (cos_lat0 * cos(amax / kRad2Deg));
+    if (!(q < 1.0)) return 180.0;
+    return 2.0 * asin(sqrt(q)) * kRad2Deg * (1.0 + 1e-9) + 1e-6;
+  };
 
   int64_t nin = 0, ncand = 0;
   double lsum = 0.0;          // kFillDiag: this lane's weights and hashes
@@ -266,7 +273,21 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     qn -= cnt;
   };
 
+  // lane i bounds band b0 + i; bands past the 64th (caps wider than 32
+  // degrees) take whole longitude circles
+  s_dl[wid][lane] = b0 + lane <= b1 ? band_halfwidth(b0 + lane) : 180.0;
+  wave_lds_fence();
   for (int b = b0; b <= b1; ++b) {
+    const double dl = b - b0 < 64 ? s_dl[wid][b - b0] : 180.0;
+    if (dl < 0.0) continue;
+    int nr = 1;
+    int rc0[2] = {0, 0}, rc1[2] = {kLonCells - 1, kLonCells - 1};
+    if (dl < 180.0) {
+      const int c0 = (int)floor(lon0 - dl), c1 = (int)floor(lon0 + dl);
+      if (c0 < 0) { nr = 2; rc0[0] = c0 + kLonCells; rc1[0] = kLonCells - 1; rc0[1] = 0; rc1[1] = c1; }
+      else if (c1 >= kLonCells) { nr = 2; rc0[0] = c0; rc1[0] = kLonCells - 1; rc0[1] = 0; rc1[1] = c1 - kLonCells; }
+      else { rc0[0] = c0; rc1[0] = c1; }
+    }
     for (int r = 0; r < nr; ++r) {
       const int64_t sb = O.cell_start[b * kLonCells + rc0[r]];
       const int64_t se = O.cell_start[b * kLonCells + rc1[r] + 1];
