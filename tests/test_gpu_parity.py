@@ -155,3 +155,26 @@ def test_f32_view_matches_f64():
         b = s.state_f32()
     for x, y in zip(a, b):
         assert np.array_equal(x.astype(np.float32), y)
+
+
+@pytest.mark.parametrize("loc,path,prec", [("LOC_NOVRT", "plain", "fp64"),
+                                           ("LOC_NOVRT", "batched", "bf16x3"),
+                                           ("LOC_VGAUSS", "plain", "fp64")])
+def test_max_obs_capacity(loc, path, prec):
+    """A search set larger than max_obs (src/letkf_solver.F90:165) fails the
+    run with LETKF_ECAPACITY; exactly max_obs passes.  With one localization
+    group the search-set sizes come from the fill pass, with several from
+    the count pass."""
+    from letkf_amd.solver import LetkfError, LetkfGpuSolver
+    p = synth.random_problem(seed=16, nobs=2000, loc_cls=loc, ns=4)
+    gpu = run_gpu(p, precision=prec, path=path)
+    top = int(gpu["obscount"].max())
+    print("largest search set", top, flush=True)
+    assert top > 1
+    with LetkfGpuSolver(p, precision=prec, path=path, max_obs=top - 1) as s:
+        with pytest.raises(LetkfError) as e:
+            s.run()
+        assert e.value.code == 5 and "max_obs" in str(e.value)
+    ok = run_gpu(p, precision=prec, path=path, max_obs=top)
+    assert np.array_equal(ok["obscount"], gpu["obscount"])
+    assert np.array_equal(ok["mean"], gpu["mean"])

@@ -15,9 +15,11 @@
 // apply the exact test (squared chord <= threshold) and the localizer in
 // the deterministic arithmetic of det_math.h.  Kept observations are counted
 // per localization group with ballots.  The same kernel runs twice: the
-// count pass sizes the lists (and writes the diagnostics), the fill pass
-// writes each (gridpoint, group)'s kept (obs index, weight) list in scan
-// order.
+// count pass sizes the lists, the fill pass writes each (gridpoint,
+// group)'s kept (obs index, weight) list in scan order.  With several
+// groups the count pass localizes (exact counts, diagnostics); with one it
+// only sums the cover's cell sizes, and the fill pass writes the
+// diagnostics.
 #include <hip/hip_runtime.h>
 
 #include "det_math.h"
@@ -77,10 +79,12 @@ __device__ __forceinline__ bool group_has_slabs(const LocDev& L, int g, int mld)
 // Passes (one wave per gridpoint):
 //   kCountExact  kept counts and diagnostics per group (weights evaluated)
 //   kFill        lists written at the offsets of the exact counts
-//   kCountBound  one group: search-set size only (chord test, no weights),
-//                an upper bound of the kept count used to size the list
+//   kCountBound  one group: the number of observations in the cells of the
+//                cap's cover (no per-observation work), an upper bound of
+//                the kept count used to size the list
 //   kFillDiag    one group: lists written at the bound offsets, then the kept
-//                count, the solve flag and the diagnostics
+//                count, the solve flag, the search-set size and the other
+//                diagnostics
 constexpr int kCountExact = 0, kFill = 1, kCountBound = 2, kFillDiag = 3;
 constexpr int kQueue = 128;   // per-wave candidate ring (two batches)
 
@@ -291,7 +295,8 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     for (int r = 0; r < nr; ++r) {
       const int64_t sb = O.cell_start[b * kLonCells + rc0[r]];
       const int64_t se = O.cell_start[b * kLonCells + rc1[r] + 1];
-      if (!FILL) ncand += se - sb;
+      if (PASS != kFill) ncand += se - sb;
+      if (PASS == kCountBound) continue;
       for (int64_t base = sb; base < se; base += 64) {
         const int64_t s = base + lane;
         const bool valid = s < se;
@@ -301,7 +306,6 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
         const uint64_t imask = __ballot(inside);
         if (imask == 0) continue;
         nin += __popcll(imask);
-        if (PASS == kCountBound) continue;
         // qn < 64 here, so the ring (128) has room for the whole batch
         if (inside) {
           const int e = (qh + qn + lane_rank(imask)) & (kQueue - 1);
@@ -355,8 +359,8 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     atomicAdd(c + kCtrSolves, nsolve);
     atomicAdd(c + kCtrPairs, npair);
   }
-  if (PASS == kCountBound) C.cnt[t * G] = nin;      // one group
-  if (PASS == kCountExact || PASS == kCountBound) {
+  if (PASS == kCountBound) C.cnt[t * G] = ncand;    // one group
+  if (PASS == kCountExact || PASS == kFillDiag) {
     D.obscount[ij] = (int32_t)nin;
     D.maxhz[ij] = maxhz;
     D.mld[ij] = mld;
