@@ -730,10 +730,12 @@ static void tridiag_launch(hipStream_t st, unsigned nb, size_t lds, int k, const
                            double* dd, double* ee, double* U, int* nvec, int* status,
                            bool reg) {
   // k >= 32 (K = 64): the register-resident reduction, rows padded to a
-  // multiple of 8, unless reg is false (the LDS kernel)
+  // multiple of 8 (to 50 for k = 49 and 50), unless reg is false (the LDS
+  // kernel)
+  const int kq = (k == 49 || k == 50) ? 50 : (k + 7) / 8 * 8;
   if (K == 64 && reg &&
-      launch_tridiag_reg(st, nb, (k + 7) / 8 * 8, NV, k, K, L, S, V, C, D, B, bvec, tau, dd, ee,
-                         U, nvec, status))
+      launch_tridiag_reg(st, nb, kq, NV, k, K, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec,
+                         status))
     return;
   k_tridiag<K, NV><<<nb, K, lds, st>>>(k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status);
 }

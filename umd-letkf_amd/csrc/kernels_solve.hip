@@ -358,15 +358,37 @@ __global__ __launch_bounds__(256) void k_finalize(LocDev L, StateDev S, VarsDev 
   }
   const int mld = D.mld[ij];
   const double dk = (double)k, dk1 = (double)(k - 1);
-  for (int64_t s = 0; s < ns; ++s) {
+  // a slab's inputs are loaded two slabs ahead of its post-processing, so
+  // that the loads of later slabs are in flight during the wave sums
+  struct SlabIn {
+    double x0, x1;      // transformed perturbation, or background member
+    float b0, b1;       // background member
+    float bm, bs;       // background mean and spread
+  };
+  auto fetch = [&](int64_t s) {
+    SlabIn r;
     const int g = slab_group(L, V, (int)s, mld);
     const bool solved = g >= 0 && C.solve[t * C.G + g];
-    const double xb_0 = m0 ? (double)xb0[k * s + lane] : 0.0;
-    const double xb_1 = m1 ? (double)xb0[k * s + lane + 64] : 0.0;
-    double x0 = solved ? (m0 ? xa0[k * s + lane] : 0.0) : xb_0;
-    double x1 = solved ? (m1 ? xa0[k * s + lane + 64] : 0.0) : xb_1;
-    const double bm = (double)S.mean[ns * ij + s];
-    const double bs = (double)S.sprd[ns * ij + s];
+    r.b0 = m0 ? xb0[k * s + lane] : 0.0f;
+    r.b1 = m1 ? xb0[k * s + lane + 64] : 0.0f;
+    r.x0 = solved ? (m0 ? xa0[k * s + lane] : 0.0) : (double)r.b0;
+    r.x1 = solved ? (m1 ? xa0[k * s + lane + 64] : 0.0) : (double)r.b1;
+    r.bm = S.mean[ns * ij + s];
+    r.bs = S.sprd[ns * ij + s];
+    return r;
+  };
+  SlabIn cur = fetch(0);
+  SlabIn nxt = ns > 1 ? fetch(1) : cur;
+  for (int64_t s = 0; s < ns; ++s) {
+    const SlabIn in = cur;
+    cur = nxt;
+    if (s + 2 < ns) nxt = fetch(s + 2);
+    const double xb_0 = (double)in.b0;
+    const double xb_1 = (double)in.b1;
+    double x0 = in.x0;
+    double x1 = in.x1;
+    const double bm = (double)in.bm;
+    const double bs = (double)in.bs;
     // recenter (:299-303)
     x0 += bm; x1 += bm;
     double mean = wsum((m0 ? x0 : 0.0) + (m1 ? x1 : 0.0)) / dk;

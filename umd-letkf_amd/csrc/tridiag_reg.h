@@ -12,7 +12,7 @@ namespace lk {
 
 // ---------------------------------------------------------------------------
 // The same reduction with the matrix in registers: lane r keeps row r of A
-// (a[c] = A(r, c), KQ >= k a multiple of 8).  The Householder vector v goes
+// (a[c] = A(r, c), KQ >= k even).  The Householder vector v goes
 // through LDS for the symmetric product A v (read by all lanes), and the
 // rank-2 update takes v_c and w_c = (t A v - (t^2 v.Av / 2) v)_c from lane c
 // by readlane, so the LDS and the VALU share the broadcast work.  The step

@@ -41,7 +41,7 @@ def hipcc() -> str:
 # step loop fully unrolled (static register indices), beyond LLVM's default
 # size limit for unroll pragmas
 EXTRA = {f"tridiag_reg_kq{kq}.hip": ["-mllvm", "-pragma-unroll-threshold=2000000"]
-         for kq in (32, 40, 48, 56, 64)}
+         for kq in (32, 40, 48, 50, 56, 64)}
 
 
 def _compile(src: str) -> str:
