@@ -650,13 +650,15 @@ void k_ratsolve(int k, int K, int NV, int64_t np, ChunkDev C, const double* __re
         if (i < k) wx += ux[i] * p[i];
     }
     wx = __shfl(wx, kRatLanes - 1, kRatLanes);
-    // y_i = sum over node lanes of coef * x_i
+    // y_i = sum over node lanes of coef * x_i (rows past k stay unused)
 #pragma unroll
     for (int i = 0; i < KQ; ++i) {
-      double t = coef * p[i];
+      if (i < k) {
+        double t = coef * p[i];
 #pragma unroll
-      for (int o = kRatLanes / 2; o >= 1; o >>= 1) t += __shfl_xor(t, o, kRatLanes);
-      p[i] = t;
+        for (int o = kRatLanes / 2; o >= 1; o >>= 1) t += __shfl_xor(t, o, kRatLanes);
+        p[i] = t;
+      }
     }
     if (act) {
       double* uo = U + (pair * NV + q) * K;
@@ -768,7 +770,8 @@ static void pipeline(hipStream_t st, const LocDev& L, const StateDev& S, const V
   if (NV > 0 && R.nb > 0) {
     // rational T^(-1/2): no QL, no eigenvectors
     const unsigned nr = (unsigned)((np + 1) / 2);
-    switch ((k + 7) / 8) {
+    // rows of 50 for k = 49 and 50 (as the register tridiagonalization)
+    switch (k == 49 || k == 50 ? 50 : (k + 7) / 8) {
       case 1: ratsolve_launch<8>(st, nr, k, K, NV, np, C, dd, ee, nvec, status, R, U); break;
       case 2: ratsolve_launch<16>(st, nr, k, K, NV, np, C, dd, ee, nvec, status, R, U); break;
       case 3: ratsolve_launch<24>(st, nr, k, K, NV, np, C, dd, ee, nvec, status, R, U); break;
@@ -776,6 +779,7 @@ static void pipeline(hipStream_t st, const LocDev& L, const StateDev& S, const V
       case 5: ratsolve_launch<40>(st, nr, k, K, NV, np, C, dd, ee, nvec, status, R, U); break;
       case 6: ratsolve_launch<48>(st, nr, k, K, NV, np, C, dd, ee, nvec, status, R, U); break;
       case 7: ratsolve_launch<56>(st, nr, k, K, NV, np, C, dd, ee, nvec, status, R, U); break;
+      case 50: ratsolve_launch<50>(st, nr, k, K, NV, np, C, dd, ee, nvec, status, R, U); break;
       default: ratsolve_launch<64>(st, nr, k, K, NV, np, C, dd, ee, nvec, status, R, U); break;
     }
     k_qapply<K><<<nb, K, 0, st>>>(L, S, V, C, D, NV, B, tau, nvec, status, U, ana);
