@@ -44,4 +44,17 @@ __device__ __forceinline__ double wave_sum_dpp(double x) {
   return __hiloint2double(hi, lo);
 }
 
+// sum over each 32-lane half of a wave, left in the half's last lane (31 or
+// 63; the other lanes hold partial sums): row_shr 1, 2, 4, 8 leave each
+// 16-lane row's sum in its lane 15, row_bcast 15 adds row 0 into row 1 and
+// row 2 into row 3.  The whole wave must be active.
+__device__ __forceinline__ double half_sum_dpp(double x) {
+  x += dpp_d<0x111, 0xf>(x);
+  x += dpp_d<0x112, 0xf>(x);
+  x += dpp_d<0x114, 0xf>(x);
+  x += dpp_d<0x118, 0xf>(x);
+  x += dpp_d<0x142, 0xa>(x);
+  return x;
+}
+
 }  // namespace lk

@@ -649,23 +649,18 @@ void k_ratsolve(int k, int K, int NV, int64_t np, ChunkDev C, const double* __re
       for (int i = 0; i < KQ; ++i)
         if (i < k) wx += ux[i] * p[i];
     }
-    wx = __shfl(wx, kRatLanes - 1, kRatLanes);
-    // y_i = sum over node lanes of coef * x_i (rows past k stay unused)
+    // y_i = sum over node lanes of coef * x_i, gathered by DPP in the last
+    // lane of each half (the T^-1 b lane, whose coef is 0); rows past k
+    // stay unused
 #pragma unroll
-    for (int i = 0; i < KQ; ++i) {
-      if (i < k) {
-        double t = coef * p[i];
-#pragma unroll
-        for (int o = kRatLanes / 2; o >= 1; o >>= 1) t += __shfl_xor(t, o, kRatLanes);
-        p[i] = t;
-      }
-    }
-    if (act) {
+    for (int i = 0; i < KQ; ++i)
+      if (i < k) p[i] = half_sum_dpp(coef * p[i]);
+    if (act && !node) {
       double* uo = U + (pair * NV + q) * K;
 #pragma unroll
       for (int i = 0; i < KQ; ++i)
-        if (i < k && (i & (kRatLanes - 1)) == j) uo[i] = p[i];
-      if (j == 0) uo[K - 1] = wx;
+        if (i < k) uo[i] = p[i];
+      uo[K - 1] = wx;
     }
   }
 }
