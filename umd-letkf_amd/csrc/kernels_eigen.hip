@@ -694,11 +694,15 @@ __global__ __launch_bounds__(64) void k_qapply(LocDev L, StateDev S, VarsDev V, 
     const double* up = U + (pair * NV + q) * K;
     double y = r < k ? up[r] : 0.0;
     const double wx = up[K - 1];
+    // reflector jj - 1 is loaded while reflector jj is applied
+    auto refl = [&](int jj) { return (r > jj && r < k) ? Bp[(int64_t)jj * k + r] : 0.0; };
+    double hn = k >= 2 ? refl(k - 2) : 0.0;
     for (int jj = k - 2; jj >= 0; --jj) {
+      const double h = hn;
+      if (jj > 0) hn = refl(jj - 1);
       const double tj = tp[jj];
       if (tj == 0.0) continue;
-      const double h = (r > jj && r < k) ? Bp[(int64_t)jj * k + r] : 0.0;
-      const double dot = wsumK<K>(h * y);
+      const double dot = K == 64 ? wave_sum_dpp(h * y) : wsumK<K>(h * y);
       y -= (tj * dot) * h;
     }
     if (r < k) ana[(int64_t)S.k * S.ns * ij + (int64_t)k * s + r] = y + wx;
