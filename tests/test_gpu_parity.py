@@ -175,6 +175,8 @@ def test_max_obs_capacity(loc, path, prec):
         with pytest.raises(LetkfError) as e:
             s.run()
         assert e.value.code == 5 and "max_obs" in str(e.value)
+        with pytest.raises(LetkfError):      # no analysis to fetch
+            s.state_f64()
     ok = run_gpu(p, precision=prec, path=path, max_obs=top)
     assert np.array_equal(ok["obscount"], gpu["obscount"])
     assert np.array_equal(ok["mean"], gpu["mean"])

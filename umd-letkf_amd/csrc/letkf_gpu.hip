@@ -910,21 +910,22 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     // list space: padded offsets, plus one alignment unit of slack for the
     // vector loads past the last list
     const int64_t total = b_tot[0] + PadCount()(b_tot[1]) + kListAlign;
-    // search-set sizes: counted by the count pass with several groups, by
-    // the fill pass with one (its count pass only bounds them by the cover)
-    auto over_capacity = [&]() {
+    // search-set sizes: counted by the count pass with several groups
+    // (checked here, before any analysis is written), by the fill pass with
+    // one (its count pass only bounds them by the cover; checked once the
+    // run's work is done)
+    if (G > 1) {
       unsigned long long maxc = 0;
       for (int sl = 0; sl < kCtrSlots; ++sl)
         maxc = std::max(maxc, h->h_ctr[sl * kCtrStride + kCtrMax]);
-      if ((long long)maxc <= (long long)h->dims.max_obs) return false;
-      hipStreamSynchronize(st);
-      hipStreamSynchronize(h->stream2);
-      cleanup_events();
-      fail(h, LETKF_ECAPACITY, "a gridpoint has %llu observations in its search radius, "
-           "above max_obs = %lld", maxc, (long long)h->dims.max_obs);
-      return true;
-    };
-    if (G > 1 && over_capacity()) return LETKF_ECAPACITY;
+      if ((long long)maxc > (long long)h->dims.max_obs) {
+        hipStreamSynchronize(st);
+        hipStreamSynchronize(h->stream2);
+        cleanup_events();
+        return fail(h, LETKF_ECAPACITY, "a gridpoint has %llu observations in its search "
+                    "radius, above max_obs = %lld", maxc, (long long)h->dims.max_obs);
+      }
+    }
     if (total > b_list_cap) {
       // this set's previous chunk is finished (the search stream waited for it)
       const int64_t cap = total + total / 4 + 1024;
@@ -936,11 +937,6 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     }
     launch_search(sst, true, h->loc, O, S, V, C, D);
     RUNCHK(hipGetLastError());
-    if (G == 1) {
-      RUNCHK(hipMemcpyAsync(h->h_ctr, h->d_ctr, ctr_bytes, hipMemcpyDeviceToHost, sst));
-      RUNCHK(hipStreamSynchronize(sst));
-      if (over_capacity()) return LETKF_ECAPACITY;
-    }
     RUNCHK(hipEventRecord(ce[1], sst));
     if (overlap) RUNCHK(hipStreamWaitEvent(st, ce[1], 0));
     RUNCHK(hipEventRecord(ce[2], st));
@@ -1011,10 +1007,18 @@ int letkf_gpu_run(letkf_gpu_t* h) {
   {
     HIPCHK(h, hipMemcpyAsync(h->h_ctr, h->d_ctr, ctr_bytes, hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipStreamSynchronize(st));
+    unsigned long long maxc = 0;
     for (int sl = 0; sl < kCtrSlots; ++sl) {
       T.n_problems += (int64_t)h->h_ctr[sl * kCtrStride + kCtrSolves];
       T.n_cand += (int64_t)h->h_ctr[sl * kCtrStride + kCtrCand];
       T.n_pairs += (int64_t)h->h_ctr[sl * kCtrStride + kCtrPairs];
+      maxc = std::max(maxc, h->h_ctr[sl * kCtrStride + kCtrMax]);
+    }
+    if ((long long)maxc > (long long)h->dims.max_obs) {
+      // the analysis buffers hold this run's unusable result
+      h->have_run = false;
+      return fail(h, LETKF_ECAPACITY, "a gridpoint has %llu observations in its search "
+                  "radius, above max_obs = %lld", maxc, (long long)h->dims.max_obs);
     }
   }
   h->timing = T;
