@@ -297,11 +297,17 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
       const int64_t se = O.cell_start[b * kLonCells + rc1[r] + 1];
       if (PASS != kFill) ncand += se - sb;
       if (PASS == kCountBound) continue;
+      // each batch's positions are loaded while the previous one is tested
+      double px = 0.0, py = 0.0, pz = 0.0;
+      if (sb + lane < se) { px = O.x[sb + lane]; py = O.y[sb + lane]; pz = O.z[sb + lane]; }
       for (int64_t base = sb; base < se; base += 64) {
         const int64_t s = base + lane;
         const bool valid = s < se;
+        const double cx = px, cy = py, cz = pz;
+        const int64_t sn = s + 64;
+        if (sn < se) { px = O.x[sn]; py = O.y[sn]; pz = O.z[sn]; }
         double c2 = 9.0;
-        if (valid) c2 = dm::chord2(O.x[s], O.y[s], O.z[s], gx, gy, gz);
+        if (valid) c2 = dm::chord2(cx, cy, cz, gx, gy, gz);
         const bool inside = c2 <= thr2;
         const uint64_t imask = __ballot(inside);
         if (imask == 0) continue;
