@@ -36,8 +36,10 @@ def test_config1(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16"])
-@pytest.mark.parametrize("k", [20, 50])
+@pytest.mark.parametrize("k", [20, 49, 50])
 def test_random(prec, k):
+    """k = 49 and 50 take the 50-row tridiagonalization and rational solve
+    (49 with one padding row)."""
     _mixed(synth.random_problem(seed=200 + k, k=k, ns=2, nobs=4000, rtps=0.5), prec)
 
 

@@ -58,7 +58,7 @@ def test_rtpp_mul_k50():
     _check(p)
 
 
-@pytest.mark.parametrize("k", [20, 50, 63])
+@pytest.mark.parametrize("k", [20, 49, 50, 63])
 def test_jacobi_and_ql_solvers_agree(k, monkeypatch):
     """The tridiagonal-QL pipeline and the Jacobi solver (the fallback for
     problems the QL cannot finish) both meet the 1e-10 bar."""

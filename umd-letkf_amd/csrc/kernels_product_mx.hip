@@ -28,7 +28,8 @@ namespace lk {
 
 template <int NB>
 struct MxFrag {
-  float y[NB][8];   // sqrt(rloc_j/err_j) y_j[32 rb + lane%32], the lane's 8 obs
+  float y[NB][8];   // y_j[32 rb + lane%32] of the lane's 8 obs, as loaded
+  float w[8];       // their sqrt(rloc_j/err_j), applied by mx_step
 };
 
 // observation e (0..7) of a lane in a 16-observation step: the BF16 MFMA
@@ -83,18 +84,25 @@ __device__ __forceinline__ void load_cols(MxFrag<NB>& F, __amdgpu_buffer_rsrc_t 
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const uint32_t vo = Lr.sidx[e] + 4u * (uint32_t)i;
-    const float ww = __uint_as_float(Lr.wb[e]);
+    F.w[e] = __uint_as_float(Lr.wb[e]);
 #pragma unroll
     for (int rb = 0; rb < NB; ++rb)
-      F.y[rb][e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(hxr, vo, 128 * rb, 0)) * ww;
+      F.y[rb][e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(hxr, vo, 128 * rb, 0));
   }
 }
 
 // acc += the products of one 16-observation step
 template <int NB, int PREC>
-__device__ __forceinline__ void mx_step(f32x16* acc, const MxFrag<NB>& cur) {
+__device__ __forceinline__ void mx_step(f32x16* acc, const MxFrag<NB>& in) {
   // both operands are sqrt(R^-1 rloc) y: A B^T = Yb^T R^-1 Yb, and row k
-  // (sqrt(R^-1 rloc) d) gives Yb^T R^-1 d in column k
+  // (sqrt(R^-1 rloc) d) gives Yb^T R^-1 d in column k.  The weights are
+  // applied here, where the columns are needed, and not next to their
+  // loads, so that no wait for the loads is placed ahead of this step.
+  MxFrag<NB> cur;
+#pragma unroll
+  for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) cur.y[rb][e] = in.y[rb][e] * in.w[e];
   if (PREC == LETKF_PREC_FP32) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -164,15 +172,24 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
   ListRegs la, lb;
   load_list<PREC>(la, ir, wr, 0, lane);
   load_list<PREC>(lb, ir, wr, 16, lane);
+  __builtin_amdgcn_sched_barrier(0);
   load_cols<NB>(fa, hxr, la, lane);
+  __builtin_amdgcn_sched_barrier(0);
   load_list<PREC>(la, ir, wr, 32, lane);
+  __builtin_amdgcn_sched_barrier(0);
+  // scheduling barriers keep each step's loads ahead of the other buffer's
+  // products, so the waits before a step cover only that step's loads
   for (int j0 = 0; j0 < m; j0 += 32) {
     load_cols<NB>(fb, hxr, lb, lane);
     load_list<PREC>(lb, ir, wr, j0 + 48, lane);
+    __builtin_amdgcn_sched_barrier(0);
     mx_step<NB, PREC>(acc, fa);
+    __builtin_amdgcn_sched_barrier(0);
     load_cols<NB>(fa, hxr, la, lane);
     load_list<PREC>(la, ir, wr, j0 + 64, lane);
+    __builtin_amdgcn_sched_barrier(0);
     mx_step<NB, PREC>(acc, fb);
+    __builtin_amdgcn_sched_barrier(0);
   }
 
   // 32x32 f32 accumulator: register r of lane l holds row 8(r/4) + 4(l/32) + r%4
