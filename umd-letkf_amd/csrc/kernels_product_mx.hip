@@ -47,44 +47,62 @@ __device__ __forceinline__ int obs_slot(int half, int e) {
 // row k and zeros above it, so no row or range test is needed.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// (column byte offset, weight bits) of the lane's 8 observations of a step
+// column byte offsets of the lane's 8 observations of a step
 struct ListRegs {
-  uint32_t sidx[8], wb[8];
+  uint32_t sidx[8];
 };
 
+// byte offset in the list of the lane's observation e of the step at j0
 template <int PREC>
-__device__ __forceinline__ void load_list(ListRegs& Lr, __amdgpu_buffer_rsrc_t ir,
-                                          __amdgpu_buffer_rsrc_t wr, int j0, int lane) {
+__device__ __forceinline__ uint32_t list_off(int j0, int half, int e) {
+  return (uint32_t)(j0 + obs_slot<PREC>(half, e)) * 4u;
+}
+
+template <int PREC>
+__device__ __forceinline__ void load_list(ListRegs& Lr, __amdgpu_buffer_rsrc_t ir, int j0,
+                                          int lane) {
   const int half = lane >> 5;
   if (PREC == LETKF_PREC_FP32) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const uint32_t vo = (uint32_t)(j0 + obs_slot<PREC>(half, e)) * 4u;
-      Lr.sidx[e] = __builtin_amdgcn_raw_buffer_load_b32(ir, vo, 0, 0);
-      Lr.wb[e] = __builtin_amdgcn_raw_buffer_load_b32(wr, vo, 0, 0);
-    }
+    for (int e = 0; e < 8; ++e)
+      Lr.sidx[e] = __builtin_amdgcn_raw_buffer_load_b32(ir, list_off<PREC>(j0, half, e), 0, 0);
   } else {
-    const uint32_t vo = (uint32_t)(j0 + 8 * half) * 4u;
+    const uint32_t vo = list_off<PREC>(j0, half, 0);
     const u32x4 a0 = __builtin_amdgcn_raw_buffer_load_b128(ir, vo, 0, 0);
     const u32x4 a1 = __builtin_amdgcn_raw_buffer_load_b128(ir, vo, 16, 0);
-    const u32x4 b0 = __builtin_amdgcn_raw_buffer_load_b128(wr, vo, 0, 0);
-    const u32x4 b1 = __builtin_amdgcn_raw_buffer_load_b128(wr, vo, 16, 0);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      Lr.sidx[e] = a0[e]; Lr.sidx[4 + e] = a1[e];
-      Lr.wb[e] = b0[e]; Lr.wb[4 + e] = b1[e];
+      Lr.sidx[e] = a0[e];
+      Lr.sidx[4 + e] = a1[e];
     }
   }
 }
 
-template <int NB>
+// the step's columns (at the list's offsets) and its weights, which are
+// loaded here rather than with the offsets so that no copy of them has to
+// outlive the offsets' registers
+template <int NB, int PREC>
 __device__ __forceinline__ void load_cols(MxFrag<NB>& F, __amdgpu_buffer_rsrc_t hxr,
-                                          const ListRegs& Lr, int lane) {
-  const int i = lane & 31;
+                                          __amdgpu_buffer_rsrc_t wr, const ListRegs& Lr,
+                                          int j0, int lane) {
+  const int i = lane & 31, half = lane >> 5;
+  if (PREC == LETKF_PREC_FP32) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      F.w[e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(wr, list_off<PREC>(j0, half, e), 0, 0));
+  } else {
+    const uint32_t vo = list_off<PREC>(j0, half, 0);
+    const u32x4 b0 = __builtin_amdgcn_raw_buffer_load_b128(wr, vo, 0, 0);
+    const u32x4 b1 = __builtin_amdgcn_raw_buffer_load_b128(wr, vo, 16, 0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      F.w[e] = __uint_as_float(b0[e]);
+      F.w[4 + e] = __uint_as_float(b1[e]);
+    }
+  }
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const uint32_t vo = Lr.sidx[e] + 4u * (uint32_t)i;
-    F.w[e] = __uint_as_float(Lr.wb[e]);
 #pragma unroll
     for (int rb = 0; rb < NB; ++rb)
       F.y[rb][e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(hxr, vo, 128 * rb, 0));
@@ -170,23 +188,23 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
   // and add nothing
   MxFrag<NB> fa, fb;
   ListRegs la, lb;
-  load_list<PREC>(la, ir, wr, 0, lane);
-  load_list<PREC>(lb, ir, wr, 16, lane);
+  load_list<PREC>(la, ir, 0, lane);
+  load_list<PREC>(lb, ir, 16, lane);
   __builtin_amdgcn_sched_barrier(0);
-  load_cols<NB>(fa, hxr, la, lane);
+  load_cols<NB, PREC>(fa, hxr, wr, la, 0, lane);
   __builtin_amdgcn_sched_barrier(0);
-  load_list<PREC>(la, ir, wr, 32, lane);
+  load_list<PREC>(la, ir, 32, lane);
   __builtin_amdgcn_sched_barrier(0);
   // scheduling barriers keep each step's loads ahead of the other buffer's
   // products, so the waits before a step cover only that step's loads
   for (int j0 = 0; j0 < m; j0 += 32) {
-    load_cols<NB>(fb, hxr, lb, lane);
-    load_list<PREC>(lb, ir, wr, j0 + 48, lane);
+    load_cols<NB, PREC>(fb, hxr, wr, lb, j0 + 16, lane);
+    load_list<PREC>(lb, ir, j0 + 48, lane);
     __builtin_amdgcn_sched_barrier(0);
     mx_step<NB, PREC>(acc, fa);
     __builtin_amdgcn_sched_barrier(0);
-    load_cols<NB>(fa, hxr, la, lane);
-    load_list<PREC>(la, ir, wr, j0 + 64, lane);
+    load_cols<NB, PREC>(fa, hxr, wr, la, j0 + 32, lane);
+    load_list<PREC>(la, ir, j0 + 64, lane);
     __builtin_amdgcn_sched_barrier(0);
     mx_step<NB, PREC>(acc, fb);
     __builtin_amdgcn_sched_barrier(0);
