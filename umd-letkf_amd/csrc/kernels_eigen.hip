@@ -573,11 +573,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 // ---------------------------------------------------------------------------
 constexpr int kRatLanes = 32;        // lanes per problem: 31 nodes + the T^-1 b lane
 
+// 1/x: the hardware estimate and one Newton step, which squares its
+// relative error (the pivots are >= k - 1 > 0 and the batched path's
+// tolerance is far above the result)
 __device__ __forceinline__ double rcp_nr(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  y = y * (2.0 - x * y);
-  y = y * (2.0 - x * y);
-  return y;
+  const double y = __builtin_amdgcn_rcp(x);
+  return y * (2.0 - x * y);
 }
 
 // one wave = two problems x 32 lanes; lane j < 31 of a problem solves with
