@@ -158,7 +158,9 @@ __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev
   }
   __syncthreads();
   const int nsl = snsl;
-  if (32 * wv >= k || nsl == 0) return;
+  if (nsl == 0) return;
+  // waves past the last member only help to stage the slabs
+  const bool rows = 32 * wv < k;
   const int i = lane & 31, half = lane >> 5;
   const int ks = (k + 15) / 16;
   // T rows 32 wv + i, converted once
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int kk = 16 * st + 8 * half + e;
-      af[st][e] = (st < ks && row < k && kk < k) ? (float)Tp[kk] : 0.0f;
+      af[st][e] = (rows && st < ks && row < k && kk < k) ? (float)Tp[kk] : 0.0f;
     }
   bf16x8 ahi[KS], alo[KS];
   if (PREC != LETKF_PREC_FP32) {
@@ -183,58 +185,90 @@ __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev
   const float* xb0 = S.state + (int64_t)S.k * S.ns * ij;
   double* xa0 = ana + (int64_t)S.k * S.ns * ij;
   const bool vec = (k & 3) == 0;
-  for (int q0 = 0; q0 < nsl; q0 += 32) {
-    const int qi = q0 + i;
-    const int slab = qi < nsl ? sl[qi] : sl[0];
-    const float* xp = xb0 + (int64_t)k * slab;
-    f32x16 acc = (f32x16){};
+  // Tiles of 32 slabs are staged in LDS once for the four waves (member
+  // rows padded to 132 floats, so that the 16-byte reads of a half-wave,
+  // one slab per lane, spread over the banks), and the next tile is loaded
+  // while this one is multiplied.  Thread t stages slab t/8 of the tile,
+  // members 4 (t%8) + 32u .. +3 for u < 4, zeros past k.
+  constexpr int KPAD = 132;
+  __shared__ __align__(16) float xs[2][32][KPAD];
+  float4 pre[4];
+  auto fetch = [&](int q0) {
+    const int ql = q0 + (tid >> 3);
+    const bool sv = ql < nsl;
+    const float* xp = xb0 + (int64_t)k * (sv ? sl[ql] : sl[0]);
 #pragma unroll
-    for (int st = 0; st < KS; ++st) {
-      if (st < ks) {
-        float xf[8];
-        const int k0 = 16 * st + 8 * half;
-        if (vec && k0 + 8 <= k) {
-          // k % 4 == 0: the slab and this run of 8 members are 16-byte aligned
-          const float4 u0 = *reinterpret_cast<const float4*>(xp + k0);
-          const float4 u1 = *reinterpret_cast<const float4*>(xp + k0 + 4);
-          xf[0] = u0.x; xf[1] = u0.y; xf[2] = u0.z; xf[3] = u0.w;
-          xf[4] = u1.x; xf[5] = u1.y; xf[6] = u1.z; xf[7] = u1.w;
-        } else {
+    for (int u = 0; u < 4; ++u) {
+      const int m0 = 4 * (tid & 7) + 32 * u;
+      if (vec) {
+        pre[u] = (sv && m0 < k) ? *reinterpret_cast<const float4*>(xp + m0)
+                                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      } else {
+        pre[u].x = (sv && m0 < k) ? xp[m0] : 0.0f;
+        pre[u].y = (sv && m0 + 1 < k) ? xp[m0 + 1] : 0.0f;
+        pre[u].z = (sv && m0 + 2 < k) ? xp[m0 + 2] : 0.0f;
+        pre[u].w = (sv && m0 + 3 < k) ? xp[m0 + 3] : 0.0f;
+      }
+    }
+  };
+  auto stash = [&](int buf) {
 #pragma unroll
-          for (int e = 0; e < 8; ++e) xf[e] = k0 + e < k ? xp[k0 + e] : 0.0f;
+    for (int u = 0; u < 4; ++u)
+      *reinterpret_cast<float4*>(&xs[buf][tid >> 3][4 * (tid & 7) + 32 * u]) = pre[u];
+  };
+  const int ntile = (nsl + 31) / 32;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int tl = 0; tl < ntile; ++tl) {
+    const int cur = tl & 1;
+    const int q0 = 32 * tl;
+    if (tl + 1 < ntile) fetch(q0 + 32);
+    if (rows) {
+      const int qi = q0 + i;
+      f32x16 acc = (f32x16){};
+#pragma unroll
+      for (int st = 0; st < KS; ++st) {
+        if (st < ks) {
+          const int k0 = 16 * st + 8 * half;
+          const float4 u0 = *reinterpret_cast<const float4*>(&xs[cur][i][k0]);
+          const float4 u1 = *reinterpret_cast<const float4*>(&xs[cur][i][k0 + 4]);
+          float xf[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+          if (PREC == LETKF_PREC_FP32) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              acc = __builtin_amdgcn_mfma_f32_32x32x2f32(af[st][e], xf[e], acc, 0, 0, 0);
+          } else if (PREC == LETKF_PREC_BF16) {
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[st], to_bf16(xf), acc, 0, 0, 0);
+          } else {
+            bf16x8 xh, xl;
+            split_bf16(xf, xh, xl);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[st], xh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[st], xl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[st], xh, acc, 0, 0, 0);
+          }
         }
-        if (PREC == LETKF_PREC_FP32) {
+      }
+      // register r of lane (i, half): member 32 wv + 8 (r/4) + 4 half + r%4 of slab q0 + i
+      if (qi < nsl) {
+        double* op = xa0 + (int64_t)k * sl[qi];
 #pragma unroll
-          for (int e = 0; e < 8; ++e)
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(af[st][e], xf[e], acc, 0, 0, 0);
-        } else if (PREC == LETKF_PREC_BF16) {
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[st], to_bf16(xf), acc, 0, 0, 0);
-        } else {
-          bf16x8 xh, xl;
-          split_bf16(xf, xh, xl);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[st], xh, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[st], xl, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[st], xh, acc, 0, 0, 0);
+        for (int r4 = 0; r4 < 4; ++r4) {
+          const int mem = 32 * wv + 8 * r4 + 4 * half;       // 4 consecutive members
+          if (vec && mem + 4 <= k) {
+            double2* o2 = reinterpret_cast<double2*>(op + mem);
+            o2[0] = make_double2((double)acc[4 * r4], (double)acc[4 * r4 + 1]);
+            o2[1] = make_double2((double)acc[4 * r4 + 2], (double)acc[4 * r4 + 3]);
+          } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (mem + u < k) op[mem + u] = (double)acc[4 * r4 + u];
+          }
         }
       }
     }
-    // register r of lane (i, half): member 32 wv + 8 (r/4) + 4 half + r%4 of slab q0 + i
-    if (qi < nsl) {
-      double* op = xa0 + (int64_t)k * slab;
-#pragma unroll
-      for (int r4 = 0; r4 < 4; ++r4) {
-        const int mem = 32 * wv + 8 * r4 + 4 * half;       // 4 consecutive members
-        if (vec && mem + 4 <= k) {
-          double2* o2 = reinterpret_cast<double2*>(op + mem);
-          o2[0] = make_double2((double)acc[4 * r4], (double)acc[4 * r4 + 1]);
-          o2[1] = make_double2((double)acc[4 * r4 + 2], (double)acc[4 * r4 + 3]);
-        } else {
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            if (mem + u < k) op[mem + u] = (double)acc[4 * r4 + u];
-        }
-      }
-    }
+    if (tl + 1 < ntile) stash(cur ^ 1);
+    __syncthreads();
   }
 }
 
