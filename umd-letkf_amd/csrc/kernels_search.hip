@@ -265,9 +265,13 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
         if (keep) {
           const int64_t pos = cur + lane_rank(kmask);
           if (pos < lim) {
-            // mixed products: byte offset of the padded operand column
+            // mixed products: byte offset of the padded operand column and
+            // sqrt(w / err) by the hardware reciprocal and square root
+            // (1 ulp each; the products round far more)
             C.list_idx[pos] = C.list_mode ? (int32_t)(s * O.kp * 4) : (int32_t)s;
-            C.list_w[pos] = C.list_mode ? sqrtf(w / O.err[s]) : w;
+            C.list_w[pos] = C.list_mode
+                                ? __builtin_amdgcn_sqrtf(w * __builtin_amdgcn_rcpf(O.err[s]))
+                                : w;
           }
         }
         if (lane == 0) s_cnt[wid][g] = cur + __popcll(kmask);
