@@ -216,6 +216,12 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
   // distance, weight, hash and list work runs on full waves; lists are
   // written in the same scan order as without the queue.
   int qh = 0, qn = 0;          // ring head and length (wave-uniform)
+  // one group: the list cursor and end stay in (wave-uniform) registers
+  int64_t fd_cur = 0, fd_lim = 0;
+  if (PASS == kFillDiag) {
+    fd_cur = C.off[t];
+    fd_lim = fd_cur + C.cnt[t];
+  }
   auto localize = [&](int cnt) {
     wave_lds_fence();
     const bool inside = lane < cnt;
@@ -258,10 +264,12 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
       }
       if (FILL) {
         // writes never pass the end of this pair's list space
-        int64_t cur = 0, lim = 0;
-        if (lane == 0) { cur = s_cnt[wid][g]; lim = s_lim[wid][g]; }
-        cur = __shfl(cur, 0, 64);
-        lim = __shfl(lim, 0, 64);
+        int64_t cur = fd_cur, lim = fd_lim;
+        if (PASS != kFillDiag) {
+          if (lane == 0) { cur = s_cnt[wid][g]; lim = s_lim[wid][g]; }
+          cur = __shfl(cur, 0, 64);
+          lim = __shfl(lim, 0, 64);
+        }
         if (keep) {
           const int64_t pos = cur + lane_rank(kmask);
           if (pos < lim) {
@@ -274,7 +282,8 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
                                 : w;
           }
         }
-        if (lane == 0) s_cnt[wid][g] = cur + __popcll(kmask);
+        if (PASS == kFillDiag) fd_cur = cur + __popcll(kmask);
+        else if (lane == 0) s_cnt[wid][g] = cur + __popcll(kmask);
       }
     }
     qh = (qh + cnt) & (kQueue - 1);
@@ -335,6 +344,7 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     if (lane == 0) {
       s_sum[wid][0] = sr;
       s_hash[wid][0] = hs;
+      s_cnt[wid][0] = fd_cur;
     }
   }
   if (FILL && C.list_mode) {
