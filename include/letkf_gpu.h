@@ -235,7 +235,11 @@ int letkf_gpu_set_localizer(letkf_gpu_t* h, const letkf_loc_desc* desc);
 int letkf_gpu_set_solver(letkf_gpu_t* h, float rtps, float rtpp, float mul);
 
 /* Run the local analysis over every gridpoint of the handle.  Inputs stay
- * unchanged on the device, so run may be repeated. */
+ * unchanged on the device, so run may be repeated.  A run that fails once
+ * its work has started leaves no analysis (LETKF_ESTATE from the getters
+ * until a run succeeds).  LETKF_ECAPACITY (a search set above max_obs)
+ * comes from the chunk's count pass when the localizer has several groups,
+ * and after the run's work with one group. */
 int letkf_gpu_run(letkf_gpu_t* h);
 
 /* Analysis in the reference's single-precision layout:

@@ -820,6 +820,9 @@ int letkf_gpu_run(letkf_gpu_t* h) {
                 "has %d", h->loc.max_var_lev, h->loc.nlev);
   if (h->ns > 2048)
     return fail(h, LETKF_EUNSUPPORTED, "grid_ns %lld above 2048", (long long)h->ns);
+  // from here on the analysis buffers are overwritten: a run that fails
+  // leaves no analysis to fetch
+  h->have_run = false;
   const char* evj = getenv("LETKF_GPU_EIGEN");
   const bool mat = !(evj && strcmp(evj, "jacobi") == 0) && h->tune_mat &&
                    transform_mat_supported((int)h->k);
@@ -1015,8 +1018,6 @@ int letkf_gpu_run(letkf_gpu_t* h) {
       maxc = std::max(maxc, h->h_ctr[sl * kCtrStride + kCtrMax]);
     }
     if ((long long)maxc > (long long)h->dims.max_obs) {
-      // the analysis buffers hold this run's unusable result
-      h->have_run = false;
       return fail(h, LETKF_ECAPACITY, "a gridpoint has %llu observations in its search "
                   "radius, above max_obs = %lld", maxc, (long long)h->dims.max_obs);
     }
