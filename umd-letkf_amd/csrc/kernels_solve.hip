@@ -173,10 +173,22 @@ __global__ __launch_bounds__(256) void k_eigen_apply(LocDev L, StateDev S,
   __shared__ int s_sl[2048];
 
   // one block per problem, or, when only flagged problems are solved, a
-  // bounded grid that strides over the chunk (every thread of a block walks
-  // the same problems, so the barriers below stay uniform)
+  // bounded grid that strides over 256-problem runs of the chunk: one load
+  // per thread finds whether a run holds a flagged problem, and the block
+  // then walks that run's problems together, so the barriers below stay
+  // uniform
   const int64_t npair = C.ngp * C.G;
-  for (int64_t pair = blockIdx.x; pair < npair; pair += gridDim.x) {
+  const int64_t rstep = only_flagged ? (int64_t)gridDim.x * 256 : (int64_t)gridDim.x;
+  for (int64_t rbase = only_flagged ? (int64_t)blockIdx.x * 256 : (int64_t)blockIdx.x;
+       rbase < npair; rbase += rstep) {
+  int64_t rend = rbase + 1;
+  if (only_flagged) {
+    const int64_t mine = rbase + threadIdx.x;
+    const int fl = (mine < npair && C.solve[mine] && only_flagged[mine] != 0) ? 1 : 0;
+    if (!__syncthreads_or(fl)) continue;
+    rend = rbase + 256 < npair ? rbase + 256 : npair;
+  }
+  for (int64_t pair = rbase; pair < rend; ++pair) {
   if (!C.solve[pair]) continue;
   if (only_flagged && only_flagged[pair] == 0) continue;
   const int tid = threadIdx.x;
@@ -298,6 +310,7 @@ __global__ __launch_bounds__(256) void k_eigen_apply(LocDev L, StateDev S,
     xa0[(int64_t)k * sl + j] = acc;
   }
   __syncthreads();                            // LDS reused by the next problem
+  }
   }
 }
 
