@@ -76,16 +76,108 @@ __device__ __forceinline__ bool group_has_slabs(const LocDev& L, int g, int mld)
   return true;
 }
 
+// Search radius of a gridpoint (src/letkf_solver.F90:210-212) and the
+// localization distances it comes from.
+struct Radii {
+  float Lh, Lp, Ls, maxhz, rs;
+};
+
+__device__ __forceinline__ Radii search_radii(const LocDev& L, float lat_ij) {
+  Radii R{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (L.cls == LETKF_LOC_OCEAN) {
+    R.Lp = dm::interp(L.n_prof, L.prof_lat, L.prof_dist, lat_ij);
+    R.Ls = dm::interp(L.n_sat, L.sat_lat, L.sat_dist, lat_ij);
+    R.maxhz = R.Lp > R.Ls ? R.Lp : R.Ls;
+  } else {
+    R.Lh = dm::interp(L.n_hz, L.hz_lat, L.hz_dist, lat_ij);
+    R.maxhz = R.Lh;
+  }
+  {
+#pragma clang fp contract(off)
+    R.rs = R.maxhz * dm::kSTDEV2MAX;                  // src/letkf_solver.F90:211
+  }
+  return R;
+}
+
+// Conservative cell cover of a spherical cap: bands b0..b1 and, in each
+// band, the longitude cells the cap can reach within that band.  The count
+// pass and the fill pass use these same functions without contraction, so
+// both compute the same cells and the list space the count pass sizes holds
+// every candidate the fill pass scans.
+struct CapCover {
+  double lat0, lon0, thd, hav_cap, cos_lat0;
+  int b0, b1;
+  bool all_lon;   // the cap reaches a pole: whole bands
+};
+
+__device__ __forceinline__ CapCover cap_cover(float rs, float lat, float lon) {
+#pragma clang fp contract(off)
+  CapCover cv;
+  const double theta = (double)rs / dm::kRE * (1.0 + 1e-6) + 1e-9;
+  cv.lat0 = (double)lat;
+  cv.lon0 = (double)lon;
+  cv.lon0 = cv.lon0 - 360.0 * floor(cv.lon0 / 360.0);
+  cv.b0 = 0;
+  cv.b1 = kBands - 1;
+  cv.all_lon = true;
+  cv.thd = 180.0;
+  if (theta < 3.14159) {
+    cv.thd = theta * kRad2Deg;
+    int b0 = (int)floor(cv.lat0 - cv.thd + 90.0);
+    int b1 = (int)floor(cv.lat0 + cv.thd + 90.0);
+    cv.b0 = b0 < 0 ? 0 : (b0 >= kBands ? kBands - 1 : b0);
+    cv.b1 = b1 < 0 ? 0 : (b1 >= kBands ? kBands - 1 : b1);
+    cv.all_lon = !(cv.lat0 + cv.thd < 90.0 && cv.lat0 - cv.thd > -90.0);
+  }
+  const double sh = sin(0.5 * theta);
+  cv.hav_cap = sh * sh;
+  cv.cos_lat0 = cos(cv.lat0 / kRad2Deg);
+  return cv;
+}
+
+// longitude half-width (degrees) that bounds the cap within band b, from
+//   hav(dlon) = (hav(cap) - hav(dlat)) / (cos lat0 cos lat),
+// with dlat at its smallest over the band and |lat| at its largest over the
+// band's part of the cap's latitude range; < 0: the band misses the cap.
+// Bands past the 64th of the cover (caps wider than 32 degrees) take whole
+// longitude circles.
+__device__ __forceinline__ double cover_halfwidth(const CapCover& cv, int b) {
+#pragma clang fp contract(off)
+  if (cv.all_lon || b - cv.b0 >= 64) return 180.0;
+  const double la = (double)b - 90.0, lb = la + 1.0;
+  const double dmin = cv.lat0 < la ? la - cv.lat0 : (cv.lat0 > lb ? cv.lat0 - lb : 0.0);
+  const double sd = sin(0.5 * dmin / kRad2Deg);
+  const double num = cv.hav_cap - sd * sd;
+  if (num <= 0.0) return -1.0;
+  const double lo = fmax(la, cv.lat0 - cv.thd), hi = fmin(lb, cv.lat0 + cv.thd);
+  const double amax = lo <= hi ? fmax(fabs(lo), fabs(hi)) : fmax(fabs(la), fabs(lb));
+  const double q = num / (cv.cos_lat0 * cos(amax / kRad2Deg));
+  if (!(q < 1.0)) return 180.0;
+  return 2.0 * asin(sqrt(q)) * kRad2Deg * (1.0 + 1e-9) + 1e-6;
+}
+
+// the one or two longitude cell ranges [rc0, rc1] of half-width dl (< 180)
+__device__ __forceinline__ int cover_ranges(const CapCover& cv, double dl, int rc0[2],
+                                            int rc1[2]) {
+#pragma clang fp contract(off)
+  rc0[0] = 0; rc0[1] = 0;
+  rc1[0] = kLonCells - 1; rc1[1] = kLonCells - 1;
+  if (!(dl < 180.0)) return 1;
+  const int c0 = (int)floor(cv.lon0 - dl), c1 = (int)floor(cv.lon0 + dl);
+  if (c0 < 0) { rc0[0] = c0 + kLonCells; rc1[0] = kLonCells - 1; rc0[1] = 0; rc1[1] = c1; return 2; }
+  if (c1 >= kLonCells) { rc0[0] = c0; rc1[0] = kLonCells - 1; rc0[1] = 0; rc1[1] = c1 - kLonCells; return 2; }
+  rc0[0] = c0;
+  rc1[0] = c1;
+  return 1;
+}
+
 // Passes (one wave per gridpoint):
 //   kCountExact  kept counts and diagnostics per group (weights evaluated)
 //   kFill        lists written at the offsets of the exact counts
-//   kCountBound  one group: the number of observations in the cells of the
-//                cap's cover (no per-observation work), an upper bound of
-//                the kept count used to size the list
-//   kFillDiag    one group: lists written at the bound offsets, then the kept
-//                count, the solve flag, the search-set size and the other
-//                diagnostics
-constexpr int kCountExact = 0, kFill = 1, kCountBound = 2, kFillDiag = 3;
+//   kFillDiag    one group: lists written at the offsets of the bounds that
+//                k_count_cover (below) sizes, then the kept count, the solve
+//                flag, the search-set size and the other diagnostics
+constexpr int kCountExact = 0, kFill = 1, kFillDiag = 3;
 constexpr int kQueue = 128;   // per-wave candidate ring (two batches)
 
 // orders this wave's LDS stores and loads on either side (the queue is
@@ -124,7 +216,7 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     }
   }
   if (S.mask[ij]) {                                   // src/letkf_solver.F90:207
-    if (PASS == kCountExact || PASS == kCountBound) {
+    if (PASS == kCountExact) {
       if (lane != 0) return;
       for (int g = 0; g < G; ++g) {
         C.cnt[t * G + g] = 0;
@@ -141,20 +233,8 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
   }
 
   const float lat_ij = S.lat[ij];
-  float Lh = 0.0f, Lp = 0.0f, Ls = 0.0f, maxhz;
-  if (L.cls == LETKF_LOC_OCEAN) {
-    Lp = dm::interp(L.n_prof, L.prof_lat, L.prof_dist, lat_ij);
-    Ls = dm::interp(L.n_sat, L.sat_lat, L.sat_dist, lat_ij);
-    maxhz = Lp > Ls ? Lp : Ls;
-  } else {
-    Lh = dm::interp(L.n_hz, L.hz_lat, L.hz_dist, lat_ij);
-    maxhz = Lh;
-  }
-  float rs;
-  {
-#pragma clang fp contract(off)
-    rs = maxhz * dm::kSTDEV2MAX;                      // src/letkf_solver.F90:211
-  }
+  const Radii R = search_radii(L, lat_ij);
+  const float Lh = R.Lh, Lp = R.Lp, Ls = R.Ls, maxhz = R.maxhz, rs = R.rs;
   const double thr2 = dm::thr2(rs);
 
   int mld = 0;                                        // calc_mldlvl_bkgt
@@ -170,42 +250,8 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
 
   const double gx = S.gx[ij], gy = S.gy[ij], gz = S.gz[ij];
 
-  // conservative cell cover of the cap: bands b0..b1 and, in each band,
-  // the longitude cells the cap can reach within that band
-  const double theta = (double)rs / dm::kRE * (1.0 + 1e-6) + 1e-9;
-  const double lat0 = (double)lat_ij;
-  double lon0 = (double)S.lon[ij];
-  lon0 = lon0 - 360.0 * floor(lon0 / 360.0);
-  int b0 = 0, b1 = kBands - 1;
-  bool all_lon = true;          // the cap reaches a pole: whole bands
-  double thd = 180.0;
-  if (theta < 3.14159) {
-    thd = theta * kRad2Deg;
-    b0 = (int)floor(lat0 - thd + 90.0);
-    b1 = (int)floor(lat0 + thd + 90.0);
-    b0 = b0 < 0 ? 0 : (b0 >= kBands ? kBands - 1 : b0);
-    b1 = b1 < 0 ? 0 : (b1 >= kBands ? kBands - 1 : b1);
-    all_lon = !(lat0 + thd < 90.0 && lat0 - thd > -90.0);
-  }
-  const double sh = sin(0.5 * theta), hav_cap = sh * sh;
-  const double cos_lat0 = cos(lat0 / kRad2Deg);
-  // longitude half-width (degrees) that bounds the cap within band b, from
-  //   hav(dlon) = (hav(cap) - hav(dlat)) / (cos lat0 cos lat),
-  // with dlat at its smallest over the band and |lat| at its largest over
-  // the band's part of the cap's latitude range; < 0: the band misses the cap
-  auto band_halfwidth = [&](int b) -> double {
-    if (all_lon) return 180.0;
-    const double la = (double)b - 90.0, lb = la + 1.0;
-    const double dmin = lat0 < la ? la - lat0 : (lat0 > lb ? lat0 - lb : 0.0);
-    const double sd = sin(0.5 * dmin / kRad2Deg);
-    const double num = hav_cap - sd * sd;
-    if (num <= 0.0) return -1.0;
-    const double lo = fmax(la, lat0 - thd), hi = fmin(lb, lat0 + thd);
-    const double amax = lo <= hi ? fmax(fabs(lo), fabs(hi)) : fmax(fabs(la), fabs(lb));
-    const double q = num / (cos_lat0 * cos(amax / kRad2Deg));
-    if (!(q < 1.0)) return 180.0;
-    return 2.0 * asin(sqrt(q)) * kRad2Deg * (1.0 + 1e-9) + 1e-6;
-  };
+  const CapCover cv = cap_cover(rs, lat_ij, S.lon[ij]);
+  const int b0 = cv.b0, b1 = cv.b1;
 
   int64_t nin = 0, ncand = 0;
   double lsum = 0.0;          // kFillDiag: this lane's weights and hashes
@@ -292,24 +338,17 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
 
   // lane i bounds band b0 + i; bands past the 64th (caps wider than 32
   // degrees) take whole longitude circles
-  s_dl[wid][lane] = b0 + lane <= b1 ? band_halfwidth(b0 + lane) : 180.0;
+  s_dl[wid][lane] = b0 + lane <= b1 ? cover_halfwidth(cv, b0 + lane) : 180.0;
   wave_lds_fence();
   for (int b = b0; b <= b1; ++b) {
     const double dl = b - b0 < 64 ? s_dl[wid][b - b0] : 180.0;
     if (dl < 0.0) continue;
-    int nr = 1;
-    int rc0[2] = {0, 0}, rc1[2] = {kLonCells - 1, kLonCells - 1};
-    if (dl < 180.0) {
-      const int c0 = (int)floor(lon0 - dl), c1 = (int)floor(lon0 + dl);
-      if (c0 < 0) { nr = 2; rc0[0] = c0 + kLonCells; rc1[0] = kLonCells - 1; rc0[1] = 0; rc1[1] = c1; }
-      else if (c1 >= kLonCells) { nr = 2; rc0[0] = c0; rc1[0] = kLonCells - 1; rc0[1] = 0; rc1[1] = c1 - kLonCells; }
-      else { rc0[0] = c0; rc1[0] = c1; }
-    }
+    int rc0[2], rc1[2];
+    const int nr = cover_ranges(cv, dl, rc0, rc1);
     for (int r = 0; r < nr; ++r) {
       const int64_t sb = O.cell_start[b * kLonCells + rc0[r]];
       const int64_t se = O.cell_start[b * kLonCells + rc1[r] + 1];
       if (PASS != kFill) ncand += se - sb;
-      if (PASS == kCountBound) continue;
       // each batch's positions are loaded while the previous one is tested
       double px = 0.0, py = 0.0, pz = 0.0;
       if (sb + lane < se) { px = O.x[sb + lane]; py = O.y[sb + lane]; pz = O.z[sb + lane]; }
@@ -336,7 +375,7 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
       }
     }
   }
-  if (PASS != kCountBound && qn > 0) localize(qn);
+  if (qn > 0) localize(qn);
 
   if (PASS == kFillDiag) {
     const double sr = wave_sum_f64(lsum);
@@ -379,7 +418,6 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     atomicAdd(c + kCtrSolves, nsolve);
     atomicAdd(c + kCtrPairs, npair);
   }
-  if (PASS == kCountBound) C.cnt[t * G] = ncand;    // one group
   if (PASS == kCountExact || PASS == kFillDiag) {
     D.obscount[ij] = (int32_t)nin;
     D.maxhz[ij] = maxhz;
@@ -387,6 +425,40 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     atomicMax(c + kCtrMax, (unsigned long long)nin);
     atomicAdd(c + kCtrCand, (unsigned long long)ncand);
   }
+}
+
+// One-group count pass, one thread per gridpoint: the number of
+// observations in the cells of the cap's cover, an upper bound of the kept
+// count that sizes the list (the fill pass scans the same cells).  Masked
+// gridpoints get zero counts and diagnostics.
+__global__ __launch_bounds__(256) void k_count_cover(LocDev L, ObsDev O, StateDev S,
+                                                     ChunkDev C, DiagDev D) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= C.ngp) return;
+  const int64_t ij = C.i0 + t;
+  if (S.mask[ij]) {                                   // src/letkf_solver.F90:207
+    C.cnt[t] = 0;
+    C.solve[t] = 0;
+    D.lg_m[ij] = 0;
+    D.lg_obsloc[ij] = 0.0;
+    D.lg_hash[ij] = 0;
+    D.obscount[ij] = 0;
+    D.maxhz[ij] = 0.0f;
+    D.mld[ij] = 0;
+    return;
+  }
+  const float lat_ij = S.lat[ij];
+  const CapCover cv = cap_cover(search_radii(L, lat_ij).rs, lat_ij, S.lon[ij]);
+  int64_t ncand = 0;
+  for (int b = cv.b0; b <= cv.b1; ++b) {
+    const double dl = cover_halfwidth(cv, b);
+    if (dl < 0.0) continue;
+    int rc0[2], rc1[2];
+    const int nr = cover_ranges(cv, dl, rc0, rc1);
+    for (int r = 0; r < nr; ++r)
+      ncand += O.cell_start[b * kLonCells + rc1[r] + 1] - O.cell_start[b * kLonCells + rc0[r]];
+  }
+  C.cnt[t] = ncand;
 }
 
 void launch_search(hipStream_t st, bool fill, const LocDev& L, const ObsDev& O,
@@ -398,7 +470,7 @@ void launch_search(hipStream_t st, bool fill, const LocDev& L, const ObsDev& O,
   // the kept observations while writing them
   if (C.G == 1) {
     if (fill) k_search<kFillDiag><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
-    else k_search<kCountBound><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+    else k_count_cover<<<(unsigned)((C.ngp + 255) / 256), 256, 0, st>>>(L, O, S, C, D);
   } else {
     if (fill) k_search<kFill><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
     else k_search<kCountExact><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
