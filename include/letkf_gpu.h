@@ -144,7 +144,7 @@ typedef struct letkf_gpu_timing {
  * 32 <= k < 64, 0 the LDS kernel; overlap: 1 searches the next chunk of
  * gridpoints on a second stream while the current one is solved;
  * chunk_log2: at most 2^chunk_log2 (gridpoint, group) problems per chunk
- * (17..19).  None of them changes the analysis. */
+ * (17..20).  None of them changes the analysis. */
 typedef struct letkf_gpu_tuning {
   int32_t order;
   int32_t bucket_bits;

@@ -98,7 +98,7 @@ def test_launch_choices_and_autotune():
         st, mn, sp = s.state_f64()
         assert rel(st, st0) <= 1e-12 and rel(mn, mn0) <= 1e-12
         for bad in ((4, 0, 1), (2, 3, 1), (2, 0, 2), (2, 0, 1, 2), (2, 0, 1, 1, 16),
-                    (2, 0, 1, 1, 20)):
+                    (2, 0, 1, 1, 21)):
             with pytest.raises(LetkfError):
                 s.set_tuning(*bad)
 

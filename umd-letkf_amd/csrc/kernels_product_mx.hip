@@ -298,7 +298,7 @@ __global__ void k_bucket_keys(int mode, int bbits, int64_t np, int G, int64_t i0
   const int64_t m = solve[p] ? cnt[p] : 0;
   uint32_t key = (uint32_t)p;
   // bucket id in the top 5 + bbits bits, the position inside in the rest
-  // (27 - bbits >= 25 bits; a chunk holds at most 2^19 problems)
+  // (27 - bbits >= 25 bits; a chunk holds at most 2^20 problems)
   const int low = 27 - bbits;
   const uint32_t bucket = bucket_desc(m, bbits);
   if (mode == 1) {

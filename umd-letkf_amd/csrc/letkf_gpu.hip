@@ -129,7 +129,7 @@ struct letkf_gpu_t {
   int tune_bbits = 0;     // bucket edges: powers of two split in 2^bbits parts
   int tune_trireg = 1;    // register-resident tridiagonalization (32 <= k < 64)
   int tune_overlap = 0;   // search of the next chunk concurrent with the solve
-  int tune_chunk_log2 = 19;   // at most 2^n problems per chunk (17..19)
+  int tune_chunk_log2 = 20;   // at most 2^n problems per chunk (17..20)
   int tune_vec = 1;       // vector path when every group has few slabs
   int tune_rat = 1;       // batched vector path: rational A^(-1/2), no eigensolve
   int tune_mat = 1;       // batched, 64 <= k <= 128: explicit T applied on MFMA
@@ -214,7 +214,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   if (const char* e = getenv("LETKF_GPU_MAT")) h->tune_mat = atoi(e);
   if (const char* e = getenv("LETKF_GPU_OVERLAP")) h->tune_overlap = atoi(e) != 0;
   if (const char* e = getenv("LETKF_GPU_CHUNK_LOG2"))
-    h->tune_chunk_log2 = std::min(19, std::max(17, atoi(e)));
+    h->tune_chunk_log2 = std::min(20, std::max(17, atoi(e)));
   hipGetDevice(&h->device);
   // both streams at the default priority
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -725,7 +725,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
   HIPCHK(h, dalloc(&h->d_lgm, (size_t)(G * nij)));
   HIPCHK(h, dalloc(&h->d_lgloc, (size_t)(G * nij)));
   HIPCHK(h, dalloc(&h->d_lghash, (size_t)(G * nij)));
-  // chunk sizing: bound the per-pair work arrays to about 12 GB
+  // chunk sizing: bound the per-pair work arrays to about 32 GB
   h->use_rat = nv > 0 && h->batched() && h->tune_rat && k >= 2;
   h->cap_rot = h->cap_it = 0;
   if (h->eigK && !h->use_rat) eigen_caps((int)k, &h->cap_rot, &h->cap_it);
@@ -733,7 +733,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
                            (nv ? 8 * (int64_t)nv * h->eigK : 8 * (int64_t)h->eigK * h->eigK) +
                            16 * (int64_t)h->cap_rot + 16 * (int64_t)h->cap_it + 40;
   int64_t pairs = std::max<int64_t>(
-      1, std::min<int64_t>(1ll << h->tune_chunk_log2, (12ll << 30) / per_pair));
+      1, std::min<int64_t>(1ll << h->tune_chunk_log2, (32ll << 30) / per_pair));
   int64_t cg = std::max<int64_t>(1, pairs / G);
   cg = std::min<int64_t>(cg, std::max<int64_t>(nij, 1));
   h->chunk_gp = cg;
@@ -1199,8 +1199,8 @@ int letkf_gpu_set_tuning(letkf_gpu_t* h, const letkf_gpu_tuning* t) {
     return fail(h, LETKF_EINVAL, "tuning tridiag_reg %d not 0 or 1", t->tridiag_reg);
   if (t->overlap != 0 && t->overlap != 1)
     return fail(h, LETKF_EINVAL, "tuning overlap %d not 0 or 1", t->overlap);
-  if (t->chunk_log2 < 17 || t->chunk_log2 > 19)
-    return fail(h, LETKF_EINVAL, "tuning chunk_log2 %d outside 17..19", t->chunk_log2);
+  if (t->chunk_log2 < 17 || t->chunk_log2 > 20)
+    return fail(h, LETKF_EINVAL, "tuning chunk_log2 %d outside 17..20", t->chunk_log2);
   h->tune_bucket = t->order;
   h->tune_bbits = t->bucket_bits;
   h->tune_trireg = t->tridiag_reg;
@@ -1272,7 +1272,7 @@ int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out) {
     t.overlap = 1 - t.overlap;
     if ((rc = consider(t))) return rc;
     const letkf_gpu_tuning base = best;
-    for (int cl = 17; cl <= 19; ++cl) {
+    for (int cl = 17; cl <= 20; ++cl) {
       if (cl == base.chunk_log2) continue;
       letkf_gpu_tuning u = base;
       u.chunk_log2 = cl;

@@ -200,7 +200,7 @@ class LetkfGpuSolver:
         return {f: getattr(t, f) for f, _ in abi.Tuning._fields_}
 
     def set_tuning(self, order: int, bucket_bits: int = 0, tridiag_reg: int = 1,
-                   overlap: int = 1, chunk_log2: int = 19):
+                   overlap: int = 1, chunk_log2: int = 20):
         t = abi.Tuning(order, bucket_bits, tridiag_reg, 0, 0.0, 0.0, overlap, chunk_log2)
         self._chk(self.lib.letkf_gpu_set_tuning(self.h, C.byref(t)))
 
