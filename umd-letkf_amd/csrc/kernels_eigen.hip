@@ -687,6 +687,14 @@ __global__ __launch_bounds__(64) void k_qapply(LocDev L, StateDev S, VarsDev V, 
   const int64_t ij = C.i0 + t_gp;
   const int mld = D.mld[ij];
   const int nsl = nvec[pair];
+  // every reflector's row r and tau (lane jj holds tau_jj) are loaded once,
+  // up front, so the chain of wave sums below never waits on memory; the
+  // step loop is unrolled so that the reflectors stay in registers
+  double hv[K - 1];
+#pragma unroll
+  for (int c = 0; c < K - 1; ++c)
+    hv[c] = (c <= k - 2 && r > c && r < k) ? Bp[(int64_t)c * k + r] : 0.0;
+  const double tv = tp[r];
   int s = -1;
   for (int q = 1; q <= nsl; ++q) {
     for (++s; s < S.ns && slab_group(L, V, s, mld) != g; ++s) {
@@ -695,16 +703,13 @@ __global__ __launch_bounds__(64) void k_qapply(LocDev L, StateDev S, VarsDev V, 
     const double* up = U + (pair * NV + q) * K;
     double y = r < k ? up[r] : 0.0;
     const double wx = up[K - 1];
-    // reflector jj - 1 is loaded while reflector jj is applied
-    auto refl = [&](int jj) { return (r > jj && r < k) ? Bp[(int64_t)jj * k + r] : 0.0; };
-    double hn = k >= 2 ? refl(k - 2) : 0.0;
-    for (int jj = k - 2; jj >= 0; --jj) {
-      const double h = hn;
-      if (jj > 0) hn = refl(jj - 1);
-      const double tj = tp[jj];
+#pragma unroll
+    for (int jj = K - 2; jj >= 0; --jj) {
+      if (jj > k - 2) continue;
+      const double tj = bcast(tv, jj);
       if (tj == 0.0) continue;
-      const double dot = K == 64 ? wave_sum_dpp(h * y) : wsumK<K>(h * y);
-      y -= (tj * dot) * h;
+      const double dot = K == 64 ? wave_sum_dpp(hv[jj] * y) : wsumK<K>(hv[jj] * y);
+      y -= (tj * dot) * hv[jj];
     }
     if (r < k) ana[(int64_t)S.k * S.ns * ij + (int64_t)k * s + r] = y + wx;
   }
