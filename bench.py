@@ -403,8 +403,8 @@ def main():
                       args.problems_per_dispatch)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; "
-                          "the analysis shards by gridpoint across GPUs (see DESIGN.md)"}))
+        print(json.dumps({"error": "this build runs on one GPU only; a multi-GPU "
+                          "analysis would shard the gridpoints across GPUs"}))
         sys.exit(2)
 
     if args.config == "c5":

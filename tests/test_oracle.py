@@ -20,7 +20,7 @@ RE = 6371000.0
 # ------------------------------------------------------------ Gaspari-Cohn
 def test_gc_known_values():
     """Values of the truncated polynomial of src/letkf_loc.F90:362-376 at
-    z/c = 0, 0.5, 1, 1.5, 1.9, 2 (computed independently in the survey):
+    z/c = 0, 0.5, 1, 1.5, 1.9, 2 (evaluated by hand from the coefficients):
     1.0, 0.684895, 0.20833, 0.016473, -4.195e-5, 0."""
     L = 300e3
     c = L / math.sqrt(0.3)
