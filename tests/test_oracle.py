@@ -380,3 +380,220 @@ def test_obs_preprocess_qc_and_order():
     assert mu.tolist() == [2.0, 4.0, 6.0]
     assert pert.tolist() == [[-1.0, -1.0, -2.0], [1.0, 1.0, 2.0]]
     assert O.obs_preprocess(hx, np.ones(4))[0].size == 0
+
+
+# ------------------------------------------------------------ LOC_OCEAN, VGAUSS
+def _ocean(sat_obs, sat_plats, tloc_prof=-1.0, tloc_sat=-1.0, nlev=1, delta=0.2):
+    return synth.Localizer(
+        cls="LOC_OCEAN",
+        hzloc_prof=LinearInterpLat([0.0, 90.0], [500e3, 500e3]),
+        hzloc_sat=LinearInterpLat([0.0, 90.0], [300e3, 300e3]),
+        tloc_prof=tloc_prof, tloc_sat=tloc_sat, vtloc_surf=1, bkg_t_delta=delta,
+        bkg_t_var_idx=0, bkg_t_var_len=nlev, sat_obs=list(sat_obs),
+        sat_plats=list(sat_plats))
+
+
+# every (obsid, platid) pair of {1, 2} x {1, 2}, all at the gridpoint itself
+_PAIRS = [(1, 1), (1, 2), (2, 1), (2, 2)]
+
+
+@pytest.mark.parametrize("sat_obs,sat_plats,surface", [
+    ([], [], {(1, 1), (1, 2), (2, 1), (2, 2)}),      # both lists empty: all surface
+    ([1], [], {(1, 1), (1, 2)}),                      # obsid list only
+    ([], [2], {(1, 2), (2, 2)}),                      # platid list only
+    ([1], [2], {(1, 2)}),                             # both: AND, not OR
+])
+def test_ocean_surface_classification(sat_obs, sat_plats, surface):
+    """letkf_loc_ocean.F90:467-479: an obs is surface when it matches every
+    NON-EMPTY list (obsid in sat_obs AND platid in sat_plats); surface obs
+    get weight 0 in the SUBML group (:485-487), all others keep GC(0) = 1
+    there.  The SURF group keeps every obs at distance 0."""
+    loc = _ocean(sat_obs, sat_plats)
+    obs = [dict(obsid=a, platid=b) for a, b in _PAIRS]
+    p = synth.point_problem(loc, obs)
+    idx0, r0, _ = O.selection(p, 0, group=0)
+    idx1, r1, _ = O.selection(p, 0, group=1)
+    assert idx0.tolist() == [0, 1, 2, 3] and r0.tolist() == [1.0] * 4
+    want1 = [i for i, pr in enumerate(_PAIRS) if pr not in surface]
+    assert idx1.tolist() == want1
+    assert r1.tolist() == [1.0] * len(want1)
+
+
+def test_ocean_surface_and_profile_radii():
+    """Surface obs use the sat radius, profile obs the prof radius
+    (:489-497).  At 400 km, GC(400 km, L=300 km) and GC(400 km, L=500 km)
+    against the truncated polynomial's value at z/c = d sqrt(0.3)/L."""
+    loc = _ocean([1], [], nlev=1)
+    obs = [dict(obsid=1, dist=400e3), dict(obsid=2, dist=400e3)]
+    p = synth.point_problem(loc, obs)
+    idx, r, d = O.selection(p, 0, group=0)
+    assert idx.tolist() == [0, 1]
+    dd = float(d[0])
+    assert dd == pytest.approx(400e3, rel=1e-6)
+    assert r[0] == O.gc(dd, 300e3) and r[1] == O.gc(dd, 500e3)
+    assert r[0] == pytest.approx(_gc_np(np.array([dd]), 300e3)[0], rel=1e-6)
+    assert r[1] == pytest.approx(_gc_np(np.array([dd]), 500e3)[0], rel=1e-6)
+    assert r[0] < r[1]
+
+
+@pytest.mark.parametrize("lat,want", [(9.0, 460e3), (81.0, 300e3), (-81.0, 300e3)])
+def test_ocean_maxhz_is_max_of_tables(lat, want):
+    """loc_ocean_maxhz (letkf_loc_ocean.F90:435-444) is the MAX of the prof
+    and sat radii: prof falls 500 -> 100 km from 0 to 90 degrees (460 km at
+    9 degrees, 140 km at 81), sat is 300 km everywhere.  The search radius
+    is that times sqrt(40/3), so a profile obs at 1.5 * 460 km = 690 km is
+    in the search set at every latitude here (MIN would give 511 km at
+    81 degrees)."""
+    loc = synth.Localizer(
+        cls="LOC_OCEAN", hzloc_prof=LinearInterpLat([0.0, 90.0], [500e3, 100e3]),
+        hzloc_sat=LinearInterpLat([0.0, 90.0], [300e3, 300e3]), vtloc_surf=0,
+        sat_obs=[1], sat_plats=[])
+    p = synth.point_problem(loc, [dict(obsid=2, dist=1.5 * 460e3)], lat=lat)
+    out = O.analysis(p)
+    assert out["maxhz"][0] == pytest.approx(want, rel=1e-6)
+    assert out["obscount"][0] == 1
+
+
+@pytest.mark.parametrize("which", ["sat", "prof"])
+def test_ocean_time_localization(which):
+    """GC(obs%time, tloc) multiplies the horizontal weight when tloc > 0
+    (:490-491 surface, :496-497 profile).  Obs at the gridpoint (GC = 1) with
+    time = z_c * tloc / sqrt(0.3) for z_c = 0, 0.5, 1, 1.5 give the known
+    values 1, 0.684895, 0.20833, 0.016473; tloc <= 0 leaves weight 1."""
+    T = 6.0
+    c = T / math.sqrt(0.3)
+    zc = [0.0, 0.5, 1.0, 1.5]
+    want = [1.0, 0.684895, 0.20833, 0.016473]
+    oid = 1 if which == "sat" else 2
+    kw = dict(tloc_sat=T) if which == "sat" else dict(tloc_prof=T)
+    obs = [dict(obsid=oid, time=float(np.float32(z * c))) for z in zc]
+    p = synth.point_problem(_ocean([1], [], **kw), obs)
+    _, r, _ = O.selection(p, 0, group=0)
+    for got, v in zip(r, want):
+        assert abs(got - v) <= 2e-6 * v + 5e-8, (got, v)
+    # the other class's tloc is not applied, and negative times count by |t|
+    kw2 = dict(tloc_prof=T) if which == "sat" else dict(tloc_sat=T)
+    p2 = synth.point_problem(_ocean([1], [], **kw2), obs)
+    assert O.selection(p2, 0, group=0)[1].tolist() == [1.0] * 4
+    obs_neg = [dict(obsid=oid, time=-float(np.float32(0.5 * c)))]
+    p3 = synth.point_problem(_ocean([1], [], **kw), obs_neg)
+    assert abs(O.selection(p3, 0, group=0)[1][0] - 0.684895) < 2e-6
+
+
+@pytest.mark.parametrize("profile,mld", [
+    ([20.0, 20.1, 20.15, 19.5, 15.0, 10.0], 3),     # first |t0 - t(1+i)| > delta at i = 3
+    ([20.0, 20.5, 20.5, 20.5, 20.5, 20.5], 1),      # at i = 1
+    ([20.0, 20.1, 19.9, 20.15, 19.85, 20.0], 6),    # never: mldlvl = bkg_t_var_len
+    ([20.0, 19.9, 19.7, 19.7, 19.7, 19.7], 2),      # |.| with a falling profile
+])
+def test_ocean_mixed_layer_level(profile, mld):
+    """calc_mldlvl_bkgt (letkf_loc_ocean.F90:408-427): the first i in
+    1..len-1 with |T(1) - T(1+i)| > bkg_t_delta, else len.  Levels j <= mld
+    of every variable are SURF, the rest SUBML, and single-level variables
+    are SURF (:375-391).  With one surface obs, only SURF slabs change: the
+    SUBML slabs keep the background exactly (an empty group, :272)."""
+    nlev = 6
+    prof = list(profile) + [35.0] * nlev + [1.0]          # T, S (6 levels), SSH (1)
+    p = synth.point_problem(_ocean([], [1], nlev=nlev), [dict(platid=1, dep=0.5)],
+                            var_lev=(nlev, nlev, 1), mean_profile=prof)
+    out = O.analysis(p)
+    assert out["lg_m"][:, 0].tolist() == [1, 0]
+    changed = np.abs(out["mean"][:, 0] - p.mean[:, 0]) > 1e-4   # increments ~0.1
+    want = np.array([j < mld for j in range(nlev)] * 2 + [True])
+    assert changed.tolist() == want.tolist()
+    # a profile obs (platid 2) updates every slab
+    p2 = synth.point_problem(_ocean([], [1], nlev=nlev), [dict(platid=2, dep=0.5)],
+                             var_lev=(nlev, nlev, 1), mean_profile=prof)
+    out2 = O.analysis(p2)
+    assert out2["lg_m"][:, 0].tolist() == [1, 1]
+    assert (np.abs(out2["mean"][:, 0] - p2.mean[:, 0]) > 1e-4).all()
+
+
+def test_vgauss_single_obs_weight():
+    """LOC_VGAUSS: GC(d) * exp(-(zdim - depth_l)^2 / (2 s^2)) per level
+    group.  An obs at the gridpoint (GC = 1) and zdim 0 with levels at
+    0, s, 2s, 3s gives exp(-l^2/2) = 1, 0.606531, 0.135335, 0.011109; at
+    400 km the horizontal GC multiplies in."""
+    s = 40.0
+    depth = np.array([0.0, s, 2 * s, 3 * s], np.float32)
+    loc = synth.Localizer(cls="LOC_VGAUSS",
+                          hzloc=LinearInterpLat([0.0, 90.0], [500e3, 500e3]),
+                          vt_scale=s, lev_depth=depth)
+    p = synth.point_problem(loc, [dict(dist=0.0), dict(dist=400e3)], var_lev=(4,))
+    for l in range(4):
+        idx, r, d = O.selection(p, 0, group=l)
+        want = math.exp(-l * l / 2.0)
+        assert idx.tolist() == [0, 1]
+        assert r[0] == pytest.approx(want, rel=1e-6)
+        assert r[1] == pytest.approx(want * O.gc(float(d[1]), 500e3), rel=1e-6)
+
+
+# ------------------------------------------------------------ increment clip
+def test_increment_clip_single_obs_closed_form():
+    """ana_inc_max (src/letkf_solver.F90:308-318) clips the MEAN increment
+    only: with one obs at the gridpoint, the unclipped increment is
+    d sb^2/(sb^2 + R); clipped it is +-inc_max exactly, while the analysis
+    perturbations (and so the spread sb sqrt(R/(sb^2+R))) are untouched."""
+    for dep in (0.8, -0.8):
+        q, ij = _single_obs_problem(d=dep)
+        free = O.analysis(q, points=[ij])
+        sb2 = float((q.state[:, 0, ij].astype(np.float64) ** 2).sum() / (q.k - 1))
+        d = float(np.float32(q.oval[0] - q.hxmean[0]))
+        inc_free = free["mean"][0, 0] - q.mean[0, ij]
+        assert inc_free == pytest.approx(d * sb2 / (sb2 + 0.5), rel=1e-12)
+        cap = np.float32(0.1)
+        assert abs(inc_free) > cap
+        q.inc_max = np.array([cap], np.float32)
+        out = O.analysis(q, points=[ij])
+        inc = out["mean"][0, 0] - q.mean[0, ij]
+        assert inc == pytest.approx(math.copysign(float(cap), dep), rel=1e-12)
+        pa = out["state"][:, 0, 0] - out["mean"][0, 0]
+        pa_free = free["state"][:, 0, 0] - free["mean"][0, 0]
+        assert np.abs(pa - pa_free).max() < 1e-12
+        assert out["sprd"][0, 0] == pytest.approx(math.sqrt(sb2 * 0.5 / (sb2 + 0.5)), rel=1e-12)
+
+
+def test_bounds_on_mean_then_members_closed_form():
+    """ana_bounds clip the mean increment to bounds - x_b (:314-315), then
+    each member to bounds - x_a (:357-362) and re-mean (:364-367).  With the
+    upper bound at x_b + 0.05 below the unclipped analysis mean, the mean
+    becomes the bound, every member perturbation is clamped to <= 0, and the
+    re-mean gives members = bound + min(a y_k, 0), a = sqrt(R/(sb^2 + R))."""
+    q, ij = _single_obs_problem(d=0.8)
+    y = q.state[:, 0, ij].astype(np.float64)
+    sb2 = float((y ** 2).sum() / (q.k - 1))
+    a = math.sqrt(0.5 / (sb2 + 0.5))
+    hi = np.float32(q.mean[0, ij] + np.float32(0.05))
+    q.bounds = np.asfortranarray(np.array([[-9.99e36], [hi]], np.float32))
+    out = O.analysis(q, points=[ij])
+    want = float(hi) + np.minimum(a * y, 0.0)
+    assert np.abs(out["state"][:, 0, 0] - want).max() < 1e-12
+    assert out["mean"][0, 0] == pytest.approx(want.mean(), rel=1e-12)
+    assert out["state"][:, 0, 0].max() <= float(hi) + 1e-12
+
+
+def test_bounds_check_clip_binds():
+    """The rebuilt simple3d bounds_check: obs2's -5.0 departure on state2
+    asks for more than ana_inc_max = 2 (and state3 for more than 0.2);
+    with the clip, those gridpoints get exactly -2 (and +-0.2 before the
+    state3 member bounds re-mean), and every other gridpoint is unchanged."""
+    p = synth.simple3d("bounds_check")
+    nlev = int(p.var_lev[0])
+    s2 = slice(nlev, 2 * nlev)
+    out = O.analysis(p)
+    import copy
+    q = copy.copy(p)
+    q.inc_max = np.full(3, 9.99e36, np.float32)
+    free = O.analysis(q)
+    inc_free = free["mean"] - p.mean
+    inc = out["mean"] - p.mean
+    assert inc_free[s2].min() < -2.5                 # the clip must bind
+    assert np.abs(inc_free[2 * nlev:]).max() > 0.4
+    bind = np.abs(inc_free[s2]) > 2.0
+    assert bind.sum() >= 3
+    assert np.allclose(inc[s2][bind], 2.0 * np.sign(inc_free[s2][bind]), atol=1e-6)
+    assert np.array_equal(inc[s2][~bind], inc_free[s2][~bind])
+    # the perturbations of state2 are untouched by the clip
+    pa = out["state"][:, s2] - out["mean"][None, s2]
+    pf = free["state"][:, s2] - free["mean"][None, s2]
+    assert np.abs(pa - pf).max() < 1e-9

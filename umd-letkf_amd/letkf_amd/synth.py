@@ -524,8 +524,17 @@ def simple3d(case="default", seed=0x1E7F0100, nx=72, ny=36, nlev=4, k=50):
     nij = nx * ny
     depth = np.arange(nlev, dtype=float) * 10.0 + 1.0
     members = np.zeros((k, 3 * nlev, nij), np.float32)
+    # bounds_check: state2 and state3 get a 2.0 spread, and state3 shares
+    # 0.6 of state2's perturbation, so that obs2's -5.0 departure asks for
+    # increments beyond ana_inc_max (2 on state2, 0.2 on state3) and the
+    # clip of src/letkf_solver.F90:308-318 binds
+    sig = [0.5, 2.0, 2.0] if case == "bounds_check" else [0.5, 0.5, 0.5]
+    pv = []
     for v in range(3):
-        d = smooth_perts(g, k * nlev, lat, lon, 1000e3, 0.5).reshape(k, nlev, nij)
+        d = smooth_perts(g, k * nlev, lat, lon, 1000e3, sig[v]).reshape(k, nlev, nij)
+        if case == "bounds_check" and v == 2:
+            d = 0.6 * pv[1] + 0.8 * d
+        pv.append(d)
         members[:, v * nlev:(v + 1) * nlev] = (float(v) + d).astype(np.float32)
     state, mean, _ = ensemble_to_state(members)
     obs = SIMPLE3D_OBS[case]
@@ -605,6 +614,51 @@ def ocean_vtloc(seed=0x1E7F0200, nx=72, ny=36, nlev=6, k=10):
                         [o[8] for o in obs], [o[0] for o in obs], [o[1] for o in obs],
                         hx, [0, nlev, 2 * nlev, 3 * nlev], [nlev] * 4, loc,
                         meta=dict(case="ocean_vtloc", seed=seed))
+
+
+# ---------------------------------------------------------------- one point
+def offset_point(lat, lon, dist, bearing=0.0):
+    """(lat, lon) in degrees at great-circle distance ``dist`` (m) from
+    (lat, lon) along ``bearing`` (degrees from north)."""
+    p1, l1, b, a = (np.radians(lat), np.radians(lon), np.radians(bearing), dist / RE)
+    p2 = np.arcsin(np.sin(p1) * np.cos(a) + np.cos(p1) * np.sin(a) * np.cos(b))
+    l2 = l1 + np.arctan2(np.sin(b) * np.sin(a) * np.cos(p1), np.cos(a) - np.sin(p1) * np.sin(p2))
+    return float(np.degrees(p2)), float(np.mod(np.degrees(l2), 360.0))
+
+
+def point_problem(loc, obs, var_lev=(1,), mean_profile=None, k=10, seed=0x1E7F0300,
+                  lat=10.0, lon=200.0, inc_max=None, bounds=None):
+    """One unmasked gridpoint at (lat, lon) and a handful of observations,
+    the single-obs OBSIO_TEST pattern (src/subclass.letkf_obsio/
+    letkf_obsio_test.F90:146-220) reduced to one column.
+
+    ``obs`` is a list of dicts with keys dist (m), bearing (deg), obsid,
+    platid, zdim, time, dep (O-F), err and slab (the slab whose members are
+    the obs' hx).  ``mean_profile`` (ns,) fixes the background mean of the
+    column (the mixed-layer level reads it); every slab's perturbations are
+    one common draw, so each observation correlates with every slab."""
+    g = rng(seed)
+    ns = int(sum(var_lev))
+    base = np.zeros(ns) if mean_profile is None else np.asarray(mean_profile, float)
+    e = g.standard_normal(k)
+    e = 0.5 * (e - e.mean()) / e.std()
+    members = (base[None, :, None] + e[:, None, None]).astype(np.float32)
+    var_s = np.concatenate([[0], np.cumsum(var_lev)[:-1]]).astype(int)
+    olat, olon, cols = [], [], []
+    for o in obs:
+        la, lo = offset_point(lat, lon, o.get("dist", 0.0), o.get("bearing", 0.0))
+        olat.append(la)
+        olon.append(lo)
+        cols.append(members[:, o.get("slab", 0), 0])
+    hx = np.stack(cols, 1)
+    hxm = hx.sum(0, dtype=np.float32) / np.float32(k)
+    val = np.array([o.get("dep", 1.0) for o in obs], np.float32) + hxm
+    return make_problem(members, [lat], [lon], np.zeros(1, bool), olat, olon,
+                        [o.get("zdim", 0.0) for o in obs], [o.get("time", 0.0) for o in obs],
+                        val, [o.get("err", 1.0) for o in obs],
+                        [o.get("obsid", 1) for o in obs], [o.get("platid", 1) for o in obs],
+                        hx, var_s, list(var_lev), loc, inc_max=inc_max, bounds=bounds,
+                        meta=dict(case="point", seed=seed))
 
 
 # ---------------------------------------------------------------- random
