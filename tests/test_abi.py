@@ -54,3 +54,31 @@ def test_no_cpu_fallback_in_package():
         if f.endswith(".py"):
             src = open(os.path.join(pkg, f)).read()
             assert "import oracle" not in src and "from oracle" not in src, f
+
+
+def test_problem_validate_rejects_wrong_layout_and_dtype():
+    """Arrays reach the C ABI as raw pointers, so a C-ordered multi-
+    dimensional array or a wrong dtype must be refused before upload."""
+    import numpy as np
+    import pytest
+
+    from letkf_amd import synth
+    p = synth.random_problem(seed=1, nx=12, ny=6, nobs=40, ns=2)
+    p.validate()
+    for name, bad in [("state", np.ascontiguousarray(p.state)),
+                      ("hx", np.ascontiguousarray(p.hx)),
+                      ("mean", np.asfortranarray(p.mean, np.float64)),
+                      ("sprd", np.ascontiguousarray(p.sprd)),
+                      ("olat", p.olat.astype(np.float64)),
+                      ("mask", p.mask.astype(np.int64)),
+                      ("inc_max", p.inc_max.astype(np.float64)),
+                      ("bounds", np.ascontiguousarray(p.bounds)),
+                      ("oerr", p.oerr[:-1])]:
+        q = synth.random_problem(seed=1, nx=12, ny=6, nobs=40, ns=2)
+        setattr(q, name, bad)
+        with pytest.raises(ValueError, match=name):
+            q.validate()
+    with pytest.raises(TypeError):
+        abi.ptr(np.zeros(4), C.c_float)
+    with pytest.raises(ValueError):
+        abi.ptr(np.zeros((4, 4), np.float32)[:, ::2], C.c_float)

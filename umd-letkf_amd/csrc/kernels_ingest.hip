@@ -14,7 +14,10 @@
 //                        perturbations obs_hx - obs_hx_mean, FP32
 //  k_obs_compact         the QC compaction of :479-489 (qc > 0 dropped, order
 //                        kept) as a gather through the selected indices
+//  k_f64_to_f32          the FP32 view of the analysis (state_ij's REAL kind)
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "letkf_internal.h"
 
@@ -93,6 +96,19 @@ __global__ void k_obs_compact(int64_t nsel, int k, const int32_t* __restrict__ s
 #pragma unroll
   for (int f = 0; f < ObsFields::kI; ++f) out.i[f][j] = in.i[f][i];
   for (int m = 0; m < k; ++m) out.hx[(int64_t)k * j + m] = in.hx[(int64_t)k * i + m];
+}
+
+// FP64 -> FP32 (round to nearest), grid-stride over n values
+__global__ void k_f64_to_f32(int64_t n, const double* __restrict__ in, float* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = (float)in[i];
+}
+
+void launch_f64_to_f32(hipStream_t st, int64_t n, const double* in, float* out) {
+  if (n <= 0) return;
+  const int64_t nb = std::min<int64_t>((n + 255) / 256, 8192);
+  k_f64_to_f32<<<(unsigned)nb, 256, 0, st>>>(n, in, out);
 }
 
 void launch_state_from_members(hipStream_t st, int64_t nij, int64_t ns, int k,

@@ -12,6 +12,8 @@
 //                   (src/letkf_solver.F90:299-379)
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 
 #include "eigen_common.h"
@@ -319,21 +321,25 @@ size_t eigen_lds_bytes(int k) {
   return sizeof(double) * (2 * (size_t)n * (n + 1) + 4 * kMaxK) + sizeof(Rot) * (kMaxK / 2 + 1);
 }
 
-void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
-                            const VarsDev& V, const ChunkDev& C,
-                            const DiagDev& D, const double* B,
-                            const double* bvec, const int* only_flagged, double* ana) {
+hipError_t launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
+                                  const VarsDev& V, const ChunkDev& C,
+                                  const DiagDev& D, const double* B,
+                                  const double* bvec, const int* only_flagged, double* ana) {
   const int64_t np = C.ngp * C.G;
-  if (np <= 0) return;
+  if (np <= 0) return hipSuccess;
   const size_t lds = eigen_lds_bytes((int)S.k);
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void*)k_eigen_apply,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
-    attr = true;
-  }
+  // the dynamic LDS limit is raised once per process, thread-safely; its
+  // result is kept and returned by every later launch
+  static std::once_flag once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(once, [] {
+    attr_err = hipFuncSetAttribute((const void*)k_eigen_apply,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+  });
+  if (attr_err != hipSuccess) return attr_err;
   const unsigned nb = only_flagged ? (unsigned)std::min<int64_t>(np, 1024) : (unsigned)np;
   k_eigen_apply<<<nb, 256, lds, st>>>(L, S, V, C, D, B, bvec, only_flagged, ana);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "det_math.h"
@@ -124,6 +125,10 @@ struct letkf_gpu_t {
   int64_t* h_tot = nullptr;    // pinned
 
   letkf_gpu_timing timing{};
+  // device memory of the handle: live allocations and an optional cap
+  std::unordered_map<void*, size_t> live;
+  size_t live_bytes = 0;
+  size_t mem_limit = 0;   // 0 = none
   int tune_xcd = 0;       // XCD-contiguous block order in the product kernels
   int tune_bucket = 2;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
   int tune_bbits = 0;     // bucket edges: powers of two split in 2^bbits parts
@@ -162,15 +167,38 @@ static int fail(letkf_gpu_t* h, int code, const char* fmt, ...) {
                   __LINE__);                                                   \
   } while (0)
 
-template <typename T>
-static hipError_t dalloc(T** p, size_t n) {
-  if (*p) { hipFree(*p); *p = nullptr; }
-  if (n == 0) n = 1;
-  return hipMalloc((void**)p, n * sizeof(T));
+// Every device allocation of a handle goes through dalloc / dfree, which
+// keep the handle's live bytes.  With LETKF_GPU_MEM_LIMIT (bytes, read at
+// create) an allocation that would take them past the limit fails with
+// hipErrorOutOfMemory, as a full device does: the LETKF_ENOMEM path can be
+// exercised on a GPU with memory to spare.
+static hipError_t raw_alloc(letkf_gpu_t* h, void** p, size_t bytes) {
+  if (h->mem_limit && h->live_bytes + bytes > h->mem_limit) return hipErrorOutOfMemory;
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e == hipSuccess) {
+    h->live[*p] = bytes;
+    h->live_bytes += bytes;
+  }
+  return e;
+}
+static void raw_free(letkf_gpu_t* h, void* p) {
+  if (!p) return;
+  auto it = h->live.find(p);
+  if (it != h->live.end()) {
+    h->live_bytes -= it->second;
+    h->live.erase(it);
+  }
+  hipFree(p);
 }
 template <typename T>
-static void dfree(T*& p) {
-  if (p) hipFree(p);
+static hipError_t dalloc(letkf_gpu_t* h, T** p, size_t n) {
+  if (*p) { raw_free(h, *p); *p = nullptr; }
+  if (n == 0) n = 1;
+  return raw_alloc(h, (void**)p, n * sizeof(T));
+}
+template <typename T>
+static void dfree(letkf_gpu_t* h, T*& p) {
+  raw_free(h, p);
   p = nullptr;
 }
 
@@ -205,6 +233,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   h->k = d->ens_size;
   h->ns = d->grid_ns;
   h->nij = d->ij_count;
+  if (const char* e = getenv("LETKF_GPU_MEM_LIMIT")) h->mem_limit = strtoull(e, nullptr, 10);
   if (const char* e = getenv("LETKF_GPU_XCD")) h->tune_xcd = atoi(e);
   if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = std::min(3, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_BBITS")) h->tune_bbits = std::min(2, std::max(0, atoi(e)));
@@ -245,25 +274,25 @@ int letkf_gpu_destroy(letkf_gpu_t* h) {
   if (!h) return LETKF_OK;
   hipStreamSynchronize(h->stream);
   hipStreamSynchronize(h->stream2);
-  dfree(h->d_state); dfree(h->d_mean); dfree(h->d_sprd); dfree(h->d_lat); dfree(h->d_lon);
-  dfree(h->d_mask); dfree(h->d_gx); dfree(h->d_gy); dfree(h->d_gz);
-  dfree(h->d_ox); dfree(h->d_oy); dfree(h->d_oz); dfree(h->d_ozdim); dfree(h->d_oerr);
-  dfree(h->d_odep); dfree(h->d_otime); dfree(h->d_oidx); dfree(h->d_oobsid);
-  dfree(h->d_oplatid); dfree(h->d_ohx); dfree(h->d_ohxp); dfree(h->d_cell_start); dfree(h->d_osurf);
-  dfree(h->d_otwp); dfree(h->d_otws);
-  dfree(h->d_slab_var); dfree(h->d_slab_lev); dfree(h->d_inc_max); dfree(h->d_bounds);
-  dfree(h->d_lev_depth);
-  dfree(h->d_ana); dfree(h->d_amean); dfree(h->d_asprd); dfree(h->d_maxhz);
-  dfree(h->d_obscount); dfree(h->d_lgm); dfree(h->d_mld); dfree(h->d_ctr);
-  dfree(h->d_lgloc); dfree(h->d_lghash);
-  dfree(h->d_cnt); dfree(h->d_off); dfree(h->d_solve); dfree(h->d_list_idx);
-  dfree(h->d_list_w); dfree(h->d_B); dfree(h->d_b); dfree(h->d_tmp);
-  dfree(h->d_cnt2); dfree(h->d_off2); dfree(h->d_solve2); dfree(h->d_list_idx2);
-  dfree(h->d_list_w2); dfree(h->d_tmp2);
-  dfree(h->d_tau); dfree(h->d_dd); dfree(h->d_ee); dfree(h->d_rot); dfree(h->d_itr);
-  dfree(h->d_nit); dfree(h->d_status); dfree(h->d_Z); dfree(h->d_U); dfree(h->d_nvec);
-  dfree(h->d_rat); dfree(h->d_ratn);
-  dfree(h->d_bkey); dfree(h->d_bkey2); dfree(h->d_bval); dfree(h->d_perm); dfree(h->d_stmp);
+  dfree(h, h->d_state); dfree(h, h->d_mean); dfree(h, h->d_sprd); dfree(h, h->d_lat); dfree(h, h->d_lon);
+  dfree(h, h->d_mask); dfree(h, h->d_gx); dfree(h, h->d_gy); dfree(h, h->d_gz);
+  dfree(h, h->d_ox); dfree(h, h->d_oy); dfree(h, h->d_oz); dfree(h, h->d_ozdim); dfree(h, h->d_oerr);
+  dfree(h, h->d_odep); dfree(h, h->d_otime); dfree(h, h->d_oidx); dfree(h, h->d_oobsid);
+  dfree(h, h->d_oplatid); dfree(h, h->d_ohx); dfree(h, h->d_ohxp); dfree(h, h->d_cell_start); dfree(h, h->d_osurf);
+  dfree(h, h->d_otwp); dfree(h, h->d_otws);
+  dfree(h, h->d_slab_var); dfree(h, h->d_slab_lev); dfree(h, h->d_inc_max); dfree(h, h->d_bounds);
+  dfree(h, h->d_lev_depth);
+  dfree(h, h->d_ana); dfree(h, h->d_amean); dfree(h, h->d_asprd); dfree(h, h->d_maxhz);
+  dfree(h, h->d_obscount); dfree(h, h->d_lgm); dfree(h, h->d_mld); dfree(h, h->d_ctr);
+  dfree(h, h->d_lgloc); dfree(h, h->d_lghash);
+  dfree(h, h->d_cnt); dfree(h, h->d_off); dfree(h, h->d_solve); dfree(h, h->d_list_idx);
+  dfree(h, h->d_list_w); dfree(h, h->d_B); dfree(h, h->d_b); dfree(h, h->d_tmp);
+  dfree(h, h->d_cnt2); dfree(h, h->d_off2); dfree(h, h->d_solve2); dfree(h, h->d_list_idx2);
+  dfree(h, h->d_list_w2); dfree(h, h->d_tmp2);
+  dfree(h, h->d_tau); dfree(h, h->d_dd); dfree(h, h->d_ee); dfree(h, h->d_rot); dfree(h, h->d_itr);
+  dfree(h, h->d_nit); dfree(h, h->d_status); dfree(h, h->d_Z); dfree(h, h->d_U); dfree(h, h->d_nvec);
+  dfree(h, h->d_rat); dfree(h, h->d_ratn);
+  dfree(h, h->d_bkey); dfree(h, h->d_bkey2); dfree(h, h->d_bval); dfree(h, h->d_perm); dfree(h, h->d_stmp);
   if (h->h_tot) hipHostFree(h->h_tot);
   if (h->h_tot2) hipHostFree(h->h_tot2);
   if (h->h_ctr) hipHostFree(h->h_ctr);
@@ -280,18 +309,19 @@ int letkf_gpu_set_state_members(letkf_gpu_t* h, const float* members, const floa
   if (nij > 0 && (!members || !lat_ij || !lon_ij || !mask_ij))
     return fail(h, LETKF_EINVAL, "null state array");
   if (k < 2) return fail(h, LETKF_EINVAL, "ens_size below 2: no spread");
-  HIPCHK(h, dalloc(&h->d_state, (size_t)(k * ns * nij)));
-  HIPCHK(h, dalloc(&h->d_mean, (size_t)(ns * nij)));
-  HIPCHK(h, dalloc(&h->d_sprd, (size_t)(ns * nij)));
-  HIPCHK(h, dalloc(&h->d_lat, (size_t)nij));
-  HIPCHK(h, dalloc(&h->d_lon, (size_t)nij));
-  HIPCHK(h, dalloc(&h->d_mask, (size_t)nij));
-  HIPCHK(h, dalloc(&h->d_gx, (size_t)nij));
-  HIPCHK(h, dalloc(&h->d_gy, (size_t)nij));
-  HIPCHK(h, dalloc(&h->d_gz, (size_t)nij));
+  h->have_run = false;   // new inputs: the previous analysis is gone
+  HIPCHK(h, dalloc(h, &h->d_state, (size_t)(k * ns * nij)));
+  HIPCHK(h, dalloc(h, &h->d_mean, (size_t)(ns * nij)));
+  HIPCHK(h, dalloc(h, &h->d_sprd, (size_t)(ns * nij)));
+  HIPCHK(h, dalloc(h, &h->d_lat, (size_t)nij));
+  HIPCHK(h, dalloc(h, &h->d_lon, (size_t)nij));
+  HIPCHK(h, dalloc(h, &h->d_mask, (size_t)nij));
+  HIPCHK(h, dalloc(h, &h->d_gx, (size_t)nij));
+  HIPCHK(h, dalloc(h, &h->d_gy, (size_t)nij));
+  HIPCHK(h, dalloc(h, &h->d_gz, (size_t)nij));
   if (nij > 0) {
     float* raw = nullptr;
-    HIPCHK(h, dalloc(&raw, (size_t)(k * ns * nij)));
+    HIPCHK(h, dalloc(h, &raw, (size_t)(k * ns * nij)));
     hipError_t e = hipMemcpyAsync(raw, members, sizeof(float) * k * ns * nij,
                                   hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) {
@@ -300,7 +330,7 @@ int letkf_gpu_set_state_members(letkf_gpu_t* h, const float* members, const floa
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    dfree(raw);
+    dfree(h, raw);
     HIPCHK(h, e);
     HIPCHK(h, hipMemcpyAsync(h->d_lat, lat_ij, sizeof(float) * nij, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_lon, lon_ij, sizeof(float) * nij, hipMemcpyHostToDevice, h->stream));
@@ -320,15 +350,16 @@ int letkf_gpu_set_state(letkf_gpu_t* h, const float* state_ij, const float* mean
   const int64_t nij = h->nij, k = h->k, ns = h->ns;
   if (nij > 0 && (!state_ij || !mean_ij || !sprd_ij || !lat_ij || !lon_ij || !mask_ij))
     return fail(h, LETKF_EINVAL, "null state array");
-  HIPCHK(h, dalloc(&h->d_state, (size_t)(k * ns * nij)));
-  HIPCHK(h, dalloc(&h->d_mean, (size_t)(ns * nij)));
-  HIPCHK(h, dalloc(&h->d_sprd, (size_t)(ns * nij)));
-  HIPCHK(h, dalloc(&h->d_lat, (size_t)nij));
-  HIPCHK(h, dalloc(&h->d_lon, (size_t)nij));
-  HIPCHK(h, dalloc(&h->d_mask, (size_t)nij));
-  HIPCHK(h, dalloc(&h->d_gx, (size_t)nij));
-  HIPCHK(h, dalloc(&h->d_gy, (size_t)nij));
-  HIPCHK(h, dalloc(&h->d_gz, (size_t)nij));
+  h->have_run = false;
+  HIPCHK(h, dalloc(h, &h->d_state, (size_t)(k * ns * nij)));
+  HIPCHK(h, dalloc(h, &h->d_mean, (size_t)(ns * nij)));
+  HIPCHK(h, dalloc(h, &h->d_sprd, (size_t)(ns * nij)));
+  HIPCHK(h, dalloc(h, &h->d_lat, (size_t)nij));
+  HIPCHK(h, dalloc(h, &h->d_lon, (size_t)nij));
+  HIPCHK(h, dalloc(h, &h->d_mask, (size_t)nij));
+  HIPCHK(h, dalloc(h, &h->d_gx, (size_t)nij));
+  HIPCHK(h, dalloc(h, &h->d_gy, (size_t)nij));
+  HIPCHK(h, dalloc(h, &h->d_gz, (size_t)nij));
   if (nij > 0) {
     HIPCHK(h, hipMemcpyAsync(h->d_state, state_ij, sizeof(float) * k * ns * nij, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_mean, mean_ij, sizeof(float) * ns * nij, hipMemcpyHostToDevice, h->stream));
@@ -362,6 +393,7 @@ static int set_obs_impl(letkf_gpu_t* h, int64_t n, const float* lat, const float
   if (n > 0 && (!lat || !lon || !zdim || !time || !val || !err || !obsid || !platid ||
                 (members ? !qc : (!hx_pert || !hx_mean))))
     return fail(h, LETKF_EINVAL, "null observation array");
+  h->have_run = false;
   const int64_t k = h->k;
   // raw uploads and work arrays (freed at the end of the call)
   float *r_lat = nullptr, *r_lon = nullptr, *r_zdim = nullptr, *r_time = nullptr;
@@ -374,14 +406,14 @@ static int set_obs_impl(letkf_gpu_t* h, int64_t n, const float* lat, const float
   void* tmp = nullptr;
   int rc = LETKF_OK;
   auto cleanup = [&]() {
-    dfree(r_lat); dfree(r_lon); dfree(r_zdim); dfree(r_time); dfree(r_val); dfree(r_err);
-    dfree(r_hx); dfree(r_hxm); dfree(r_obsid); dfree(r_platid); dfree(cell); dfree(scell);
-    dfree(iota); dfree(perm); dfree(r_qc); dfree(sel); dfree(nsel_d); dfree(r_full);
-    dfree(keep);
-    for (auto& f : cf.f) dfree(f);
-    for (auto& f : cf.i) dfree(f);
-    dfree(cf.hx);
-    if (tmp) hipFree(tmp);
+    dfree(h, r_lat); dfree(h, r_lon); dfree(h, r_zdim); dfree(h, r_time); dfree(h, r_val); dfree(h, r_err);
+    dfree(h, r_hx); dfree(h, r_hxm); dfree(h, r_obsid); dfree(h, r_platid); dfree(h, cell); dfree(h, scell);
+    dfree(h, iota); dfree(h, perm); dfree(h, r_qc); dfree(h, sel); dfree(h, nsel_d); dfree(h, r_full);
+    dfree(h, keep);
+    for (auto& f : cf.f) dfree(h, f);
+    for (auto& f : cf.i) dfree(h, f);
+    dfree(h, cf.hx);
+    raw_free(h, tmp);
   };
 #define OBSCHK(call)                                                           \
   do {                                                                         \
@@ -395,10 +427,10 @@ static int set_obs_impl(letkf_gpu_t* h, int64_t n, const float* lat, const float
   } while (0)
   hipStream_t st = h->stream;
   size_t nn = (size_t)(n > 0 ? n : 1);
-  OBSCHK(dalloc(&r_lat, nn)); OBSCHK(dalloc(&r_lon, nn)); OBSCHK(dalloc(&r_zdim, nn));
-  OBSCHK(dalloc(&r_time, nn)); OBSCHK(dalloc(&r_val, nn)); OBSCHK(dalloc(&r_err, nn));
-  OBSCHK(dalloc(&r_hx, nn * k)); OBSCHK(dalloc(&r_hxm, nn)); OBSCHK(dalloc(&r_obsid, nn));
-  OBSCHK(dalloc(&r_platid, nn));
+  OBSCHK(dalloc(h, &r_lat, nn)); OBSCHK(dalloc(h, &r_lon, nn)); OBSCHK(dalloc(h, &r_zdim, nn));
+  OBSCHK(dalloc(h, &r_time, nn)); OBSCHK(dalloc(h, &r_val, nn)); OBSCHK(dalloc(h, &r_err, nn));
+  OBSCHK(dalloc(h, &r_hx, nn * k)); OBSCHK(dalloc(h, &r_hxm, nn)); OBSCHK(dalloc(h, &r_obsid, nn));
+  OBSCHK(dalloc(h, &r_platid, nn));
   if (n > 0) {
     OBSCHK(hipMemcpyAsync(r_lat, lat, 4 * n, hipMemcpyHostToDevice, st));
     OBSCHK(hipMemcpyAsync(r_lon, lon, 4 * n, hipMemcpyHostToDevice, st));
@@ -415,8 +447,8 @@ static int set_obs_impl(letkf_gpu_t* h, int64_t n, const float* lat, const float
   }
   if (members && n > 0) {
     // obs_hx_mean, perturbations and the qc <= 0 flags (src/letkf_obs.F90:459-466)
-    OBSCHK(dalloc(&r_full, nn * k)); OBSCHK(dalloc(&r_qc, nn)); OBSCHK(dalloc(&keep, nn));
-    OBSCHK(dalloc(&sel, nn)); OBSCHK(dalloc(&nsel_d, 1));
+    OBSCHK(dalloc(h, &r_full, nn * k)); OBSCHK(dalloc(h, &r_qc, nn)); OBSCHK(dalloc(h, &keep, nn));
+    OBSCHK(dalloc(h, &sel, nn)); OBSCHK(dalloc(h, &nsel_d, 1));
     OBSCHK(hipMemcpyAsync(r_full, hx_full, 4 * n * k, hipMemcpyHostToDevice, st));
     OBSCHK(hipMemcpyAsync(r_qc, qc, 4 * n, hipMemcpyHostToDevice, st));
     launch_obs_hx_split(st, n, (int)k, r_full, r_qc, r_hx, r_hxm, keep);
@@ -425,17 +457,17 @@ static int set_obs_impl(letkf_gpu_t* h, int64_t n, const float* lat, const float
     hipcub::CountingInputIterator<int32_t> idx(0);
     size_t tb = 0;
     OBSCHK(hipcub::DeviceSelect::Flagged(nullptr, tb, idx, keep, sel, nsel_d, (int)n, st));
-    OBSCHK(hipMalloc(&tmp, tb));
+    OBSCHK(raw_alloc(h, &tmp, tb));
     OBSCHK(hipcub::DeviceSelect::Flagged(tmp, tb, idx, keep, sel, nsel_d, (int)n, st));
     int32_t nsel = 0;
     OBSCHK(hipMemcpyAsync(&nsel, nsel_d, 4, hipMemcpyDeviceToHost, st));
     OBSCHK(hipStreamSynchronize(st));
-    hipFree(tmp);
+    raw_free(h, tmp);
     tmp = nullptr;
     const size_t ns1 = (size_t)(nsel > 0 ? nsel : 1);
-    for (auto& f : cf.f) OBSCHK(dalloc(&f, ns1));
-    for (auto& f : cf.i) OBSCHK(dalloc(&f, ns1));
-    OBSCHK(dalloc(&cf.hx, ns1 * k));
+    for (auto& f : cf.f) OBSCHK(dalloc(h, &f, ns1));
+    for (auto& f : cf.i) OBSCHK(dalloc(h, &f, ns1));
+    OBSCHK(dalloc(h, &cf.hx, ns1 * k));
     const ObsFields in{{r_lat, r_lon, r_zdim, r_time, r_val, r_err, r_hxm},
                        {r_obsid, r_platid}, r_hx};
     launch_obs_compact(st, nsel, (int)k, sel, in, cf);
@@ -448,20 +480,20 @@ static int set_obs_impl(letkf_gpu_t* h, int64_t n, const float* lat, const float
     nn = ns1;
   }
   h->nobs = n;
-  OBSCHK(dalloc(&h->d_ox, nn)); OBSCHK(dalloc(&h->d_oy, nn)); OBSCHK(dalloc(&h->d_oz, nn));
-  OBSCHK(dalloc(&h->d_ozdim, nn)); OBSCHK(dalloc(&h->d_oerr, nn)); OBSCHK(dalloc(&h->d_odep, nn));
-  OBSCHK(dalloc(&h->d_otime, nn)); OBSCHK(dalloc(&h->d_oidx, nn)); OBSCHK(dalloc(&h->d_oobsid, nn));
-  OBSCHK(dalloc(&h->d_oplatid, nn)); OBSCHK(dalloc(&h->d_ohx, nn * k));
-  OBSCHK(dalloc(&h->d_cell_start, (size_t)kCells + 1));
-  OBSCHK(dalloc(&h->d_osurf, nn)); OBSCHK(dalloc(&h->d_otwp, nn)); OBSCHK(dalloc(&h->d_otws, nn));
-  OBSCHK(dalloc(&cell, nn)); OBSCHK(dalloc(&scell, nn));
-  OBSCHK(dalloc(&iota, nn)); OBSCHK(dalloc(&perm, nn));
+  OBSCHK(dalloc(h, &h->d_ox, nn)); OBSCHK(dalloc(h, &h->d_oy, nn)); OBSCHK(dalloc(h, &h->d_oz, nn));
+  OBSCHK(dalloc(h, &h->d_ozdim, nn)); OBSCHK(dalloc(h, &h->d_oerr, nn)); OBSCHK(dalloc(h, &h->d_odep, nn));
+  OBSCHK(dalloc(h, &h->d_otime, nn)); OBSCHK(dalloc(h, &h->d_oidx, nn)); OBSCHK(dalloc(h, &h->d_oobsid, nn));
+  OBSCHK(dalloc(h, &h->d_oplatid, nn)); OBSCHK(dalloc(h, &h->d_ohx, nn * k));
+  OBSCHK(dalloc(h, &h->d_cell_start, (size_t)kCells + 1));
+  OBSCHK(dalloc(h, &h->d_osurf, nn)); OBSCHK(dalloc(h, &h->d_otwp, nn)); OBSCHK(dalloc(h, &h->d_otws, nn));
+  OBSCHK(dalloc(h, &cell, nn)); OBSCHK(dalloc(h, &scell, nn));
+  OBSCHK(dalloc(h, &iota, nn)); OBSCHK(dalloc(h, &perm, nn));
   if (n > 0) {
     launch_obs_prep(st, n, (int)k, r_lat, r_lon, cell, iota);
     OBSCHK(hipGetLastError());
     size_t tb = 0;
     OBSCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, cell, scell, iota, perm, (int)n, 0, 17, st));
-    OBSCHK(hipMalloc(&tmp, tb));
+    OBSCHK(raw_alloc(h, &tmp, tb));
     OBSCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, cell, scell, iota, perm, (int)n, 0, 17, st));
   }
   launch_obs_gather(st, n, (int)k, perm, r_lat, r_lon, r_zdim, r_time, r_val, r_err,
@@ -470,7 +502,7 @@ static int set_obs_impl(letkf_gpu_t* h, int64_t n, const float* lat, const float
                     h->d_oplatid, h->d_otime, h->d_ohx, h->d_cell_start);
   OBSCHK(hipGetLastError());
   h->kp = (int)(((k + 1) + 31) / 32 * 32);
-  OBSCHK(dalloc(&h->d_ohxp, nn * h->kp));
+  OBSCHK(dalloc(h, &h->d_ohxp, nn * h->kp));
   launch_obs_pad_hx(st, n, (int)k, h->kp, h->d_ohx, h->d_odep, h->d_ohxp);
   OBSCHK(hipGetLastError());
   OBSCHK(hipStreamSynchronize(st));
@@ -524,8 +556,8 @@ static int rebuild_slabs(letkf_gpu_t* h) {
     }
   }
   h->loc.max_var_lev = maxlev;
-  HIPCHK(h, dalloc(&h->d_slab_var, (size_t)h->ns));
-  HIPCHK(h, dalloc(&h->d_slab_lev, (size_t)h->ns));
+  HIPCHK(h, dalloc(h, &h->d_slab_var, (size_t)h->ns));
+  HIPCHK(h, dalloc(h, &h->d_slab_lev, (size_t)h->ns));
   HIPCHK(h, hipMemcpy(h->d_slab_var, sv.data(), 4 * h->ns, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(h->d_slab_lev, sl.data(), 4 * h->ns, hipMemcpyHostToDevice));
   return LETKF_OK;
@@ -542,11 +574,12 @@ int letkf_gpu_set_statevars(letkf_gpu_t* h, int32_t nvars, const int32_t* s_idx,
       return fail(h, LETKF_EINVAL, "variable %d: slabs [%d, %d) outside grid_ns %lld", v,
                   s_idx[v], s_idx[v] + levels[v], (long long)h->ns);
   }
+  h->have_run = false;
   h->nvars = nvars;
   h->var_s.assign(s_idx, s_idx + nvars);
   h->var_lev.assign(levels, levels + nvars);
-  HIPCHK(h, dalloc(&h->d_inc_max, (size_t)std::max(nvars, 1)));
-  HIPCHK(h, dalloc(&h->d_bounds, (size_t)std::max(2 * nvars, 1)));
+  HIPCHK(h, dalloc(h, &h->d_inc_max, (size_t)std::max(nvars, 1)));
+  HIPCHK(h, dalloc(h, &h->d_bounds, (size_t)std::max(2 * nvars, 1)));
   if (nvars > 0) {
     HIPCHK(h, hipMemcpy(h->d_inc_max, inc_max, 4 * nvars, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_bounds, bounds, 8 * nvars, hipMemcpyHostToDevice));
@@ -559,8 +592,8 @@ int letkf_gpu_set_statevars(letkf_gpu_t* h, int32_t nvars, const int32_t* s_idx,
 
 static int upload_flags(letkf_gpu_t* h) {
   int32_t *so = nullptr, *sp = nullptr;
-  HIPCHK(h, dalloc(&so, std::max<size_t>(h->sat_obs.size(), 1)));
-  HIPCHK(h, dalloc(&sp, std::max<size_t>(h->sat_plats.size(), 1)));
+  HIPCHK(h, dalloc(h, &so, std::max<size_t>(h->sat_obs.size(), 1)));
+  HIPCHK(h, dalloc(h, &sp, std::max<size_t>(h->sat_plats.size(), 1)));
   if (!h->sat_obs.empty())
     HIPCHK(h, hipMemcpy(so, h->sat_obs.data(), 4 * h->sat_obs.size(), hipMemcpyHostToDevice));
   if (!h->sat_plats.empty())
@@ -570,14 +603,15 @@ static int upload_flags(letkf_gpu_t* h) {
                    h->d_otwp, h->d_otws);
   hipError_t e = hipGetLastError();
   hipStreamSynchronize(h->stream);
-  hipFree(so);
-  hipFree(sp);
+  dfree(h, so);
+  dfree(h, sp);
   if (e != hipSuccess) return fail(h, LETKF_EHIP, "obs flags: %s", hipGetErrorString(e));
   return LETKF_OK;
 }
 
 int letkf_gpu_set_localizer(letkf_gpu_t* h, const letkf_loc_desc* d) {
   if (!h || !d) return LETKF_EINVAL;
+  h->have_run = false;
   LocDev L{};
   L.cls = d->cls;
   switch (d->cls) {
@@ -632,7 +666,7 @@ int letkf_gpu_set_localizer(letkf_gpu_t* h, const letkf_loc_desc* d) {
   h->sat_obs.assign(d->sat_obs, d->sat_obs + (d->cls == LETKF_LOC_OCEAN ? d->n_sat_obs : 0));
   h->sat_plats.assign(d->sat_plats, d->sat_plats + (d->cls == LETKF_LOC_OCEAN ? d->n_sat_plats : 0));
   if (d->cls == LETKF_LOC_VGAUSS) {
-    HIPCHK(h, dalloc(&h->d_lev_depth, (size_t)d->nlev));
+    HIPCHK(h, dalloc(h, &h->d_lev_depth, (size_t)d->nlev));
     HIPCHK(h, hipMemcpy(h->d_lev_depth, d->lev_depth, 4 * d->nlev, hipMemcpyHostToDevice));
   }
   h->loc = L;
@@ -653,6 +687,7 @@ int letkf_gpu_set_solver(letkf_gpu_t* h, float rtps, float rtpp, float mul) {
   if (rtpp != 0.0f && rtps != 0.0f)
     return fail(h, LETKF_EINVAL, "cannot have both RTPS and RTPP enabled at the same time. ");
   if (mul < 1.0f) return fail(h, LETKF_EINVAL, "infl_mul must be >=1.0.");
+  h->have_run = false;
   h->rtps = rtps;
   h->rtpp = rtpp;
   h->mul = mul;
@@ -699,13 +734,13 @@ static hipcub::TransformInputIterator<int64_t, PadCount, const int64_t*> padded_
 static int ensure_outputs(letkf_gpu_t* h) {
   const int64_t k = h->k, ns = h->ns, nij = h->nij, G = h->G;
   if (!h->d_ana) {
-    HIPCHK(h, dalloc(&h->d_ana, (size_t)(k * ns * nij)));
-    HIPCHK(h, dalloc(&h->d_amean, (size_t)(ns * nij)));
-    HIPCHK(h, dalloc(&h->d_asprd, (size_t)(ns * nij)));
-    HIPCHK(h, dalloc(&h->d_maxhz, (size_t)nij));
-    HIPCHK(h, dalloc(&h->d_obscount, (size_t)nij));
-    HIPCHK(h, dalloc(&h->d_mld, (size_t)nij));
-    HIPCHK(h, dalloc(&h->d_ctr, (size_t)kCtrSlots * kCtrStride));
+    HIPCHK(h, dalloc(h, &h->d_ana, (size_t)(k * ns * nij)));
+    HIPCHK(h, dalloc(h, &h->d_amean, (size_t)(ns * nij)));
+    HIPCHK(h, dalloc(h, &h->d_asprd, (size_t)(ns * nij)));
+    HIPCHK(h, dalloc(h, &h->d_maxhz, (size_t)nij));
+    HIPCHK(h, dalloc(h, &h->d_obscount, (size_t)nij));
+    HIPCHK(h, dalloc(h, &h->d_mld, (size_t)nij));
+    HIPCHK(h, dalloc(h, &h->d_ctr, (size_t)kCtrSlots * kCtrStride));
   }
   // LETKF_GPU_EIGEN=jacobi selects the parallel Jacobi solver for every
   // problem (reference solver for tests); default: tridiagonal QL pipeline
@@ -722,9 +757,9 @@ static int ensure_outputs(letkf_gpu_t* h) {
   h->eigK = eigK;
   h->vec_nv = nv;
   h->use_mat = use_mat;
-  HIPCHK(h, dalloc(&h->d_lgm, (size_t)(G * nij)));
-  HIPCHK(h, dalloc(&h->d_lgloc, (size_t)(G * nij)));
-  HIPCHK(h, dalloc(&h->d_lghash, (size_t)(G * nij)));
+  HIPCHK(h, dalloc(h, &h->d_lgm, (size_t)(G * nij)));
+  HIPCHK(h, dalloc(h, &h->d_lgloc, (size_t)(G * nij)));
+  HIPCHK(h, dalloc(h, &h->d_lghash, (size_t)(G * nij)));
   // chunk sizing: bound the per-pair work arrays to about 32 GB
   h->use_rat = nv > 0 && h->batched() && h->tune_rat && k >= 2;
   h->cap_rot = h->cap_it = 0;
@@ -738,51 +773,51 @@ static int ensure_outputs(letkf_gpu_t* h) {
   cg = std::min<int64_t>(cg, std::max<int64_t>(nij, 1));
   h->chunk_gp = cg;
   const int64_t np = cg * G;
-  HIPCHK(h, dalloc(&h->d_cnt, (size_t)np));
-  HIPCHK(h, dalloc(&h->d_off, (size_t)np));
-  HIPCHK(h, dalloc(&h->d_solve, (size_t)np));
-  HIPCHK(h, dalloc(&h->d_cnt2, (size_t)np));
-  HIPCHK(h, dalloc(&h->d_off2, (size_t)np));
-  HIPCHK(h, dalloc(&h->d_solve2, (size_t)np));
-  HIPCHK(h, dalloc(&h->d_B, (size_t)(np * k * k)));
-  HIPCHK(h, dalloc(&h->d_b, (size_t)(np * k)));
-  HIPCHK(h, dalloc(&h->d_status, (size_t)np));
+  HIPCHK(h, dalloc(h, &h->d_cnt, (size_t)np));
+  HIPCHK(h, dalloc(h, &h->d_off, (size_t)np));
+  HIPCHK(h, dalloc(h, &h->d_solve, (size_t)np));
+  HIPCHK(h, dalloc(h, &h->d_cnt2, (size_t)np));
+  HIPCHK(h, dalloc(h, &h->d_off2, (size_t)np));
+  HIPCHK(h, dalloc(h, &h->d_solve2, (size_t)np));
+  HIPCHK(h, dalloc(h, &h->d_B, (size_t)(np * k * k)));
+  HIPCHK(h, dalloc(h, &h->d_b, (size_t)(np * k)));
+  HIPCHK(h, dalloc(h, &h->d_status, (size_t)np));
   if (use_mat) {
-    HIPCHK(h, dalloc(&h->d_tau, (size_t)(np * 128)));
-    HIPCHK(h, dalloc(&h->d_dd, (size_t)(np * 128)));
-    HIPCHK(h, dalloc(&h->d_ee, (size_t)(np * 128)));
+    HIPCHK(h, dalloc(h, &h->d_tau, (size_t)(np * 128)));
+    HIPCHK(h, dalloc(h, &h->d_dd, (size_t)(np * 128)));
+    HIPCHK(h, dalloc(h, &h->d_ee, (size_t)(np * 128)));
   }
   if (h->eigK) {
-    HIPCHK(h, dalloc(&h->d_tau, (size_t)(np * h->eigK)));
-    HIPCHK(h, dalloc(&h->d_dd, (size_t)(np * h->eigK)));
-    HIPCHK(h, dalloc(&h->d_ee, (size_t)(np * h->eigK)));
-    HIPCHK(h, dalloc(&h->d_rot, (size_t)(np * h->cap_rot)));
-    HIPCHK(h, dalloc(&h->d_itr, (size_t)(np * h->cap_it)));
-    HIPCHK(h, dalloc(&h->d_nit, (size_t)np));
+    HIPCHK(h, dalloc(h, &h->d_tau, (size_t)(np * h->eigK)));
+    HIPCHK(h, dalloc(h, &h->d_dd, (size_t)(np * h->eigK)));
+    HIPCHK(h, dalloc(h, &h->d_ee, (size_t)(np * h->eigK)));
+    HIPCHK(h, dalloc(h, &h->d_rot, (size_t)(np * h->cap_rot)));
+    HIPCHK(h, dalloc(h, &h->d_itr, (size_t)(np * h->cap_it)));
+    HIPCHK(h, dalloc(h, &h->d_nit, (size_t)np));
     if (nv) {
-      dfree(h->d_Z);
-      HIPCHK(h, dalloc(&h->d_U, (size_t)(np * nv * h->eigK)));
-      HIPCHK(h, dalloc(&h->d_nvec, (size_t)np));
+      dfree(h, h->d_Z);
+      HIPCHK(h, dalloc(h, &h->d_U, (size_t)(np * nv * h->eigK)));
+      HIPCHK(h, dalloc(h, &h->d_nvec, (size_t)np));
     } else {
-      dfree(h->d_U);
-      HIPCHK(h, dalloc(&h->d_Z, (size_t)(np * h->eigK * h->eigK)));
+      dfree(h, h->d_U);
+      HIPCHK(h, dalloc(h, &h->d_Z, (size_t)(np * h->eigK * h->eigK)));
     }
   }
   size_t tb = 0;
   HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, padded_counts(h->d_cnt), h->d_off,
                                              (int)np, h->stream));
-  HIPCHK(h, dalloc((char**)&h->d_tmp, tb));
-  HIPCHK(h, dalloc((char**)&h->d_tmp2, tb));
+  HIPCHK(h, dalloc(h, (char**)&h->d_tmp, tb));
+  HIPCHK(h, dalloc(h, (char**)&h->d_tmp2, tb));
   h->tmp_bytes = tb;
   if (h->batched()) {
-    HIPCHK(h, dalloc(&h->d_bkey, (size_t)np));
-    HIPCHK(h, dalloc(&h->d_bkey2, (size_t)np));
-    HIPCHK(h, dalloc(&h->d_bval, (size_t)np));
-    HIPCHK(h, dalloc(&h->d_perm, (size_t)np));
+    HIPCHK(h, dalloc(h, &h->d_bkey, (size_t)np));
+    HIPCHK(h, dalloc(h, &h->d_bkey2, (size_t)np));
+    HIPCHK(h, dalloc(h, &h->d_bval, (size_t)np));
+    HIPCHK(h, dalloc(h, &h->d_perm, (size_t)np));
     size_t sb = 0;
     HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(nullptr, sb, h->d_bkey, h->d_bkey2, h->d_bval,
                                                  h->d_perm, (int)np, 0, 32, h->stream));
-    HIPCHK(h, dalloc((char**)&h->d_stmp, sb));
+    HIPCHK(h, dalloc(h, (char**)&h->d_stmp, sb));
     h->stmp_bytes = sb;
   }
   if ((h->use_rat || h->use_mat) && !h->d_rat) {
@@ -796,8 +831,8 @@ static int ensure_outputs(letkf_gpu_t* h) {
       if (n < 0) { n = 31; rat::nodes(std::ldexp(1.0, b), n, sb, sb + 32); }
       tn[b] = n;
     }
-    HIPCHK(h, dalloc(&h->d_rat, tab.size()));
-    HIPCHK(h, dalloc(&h->d_ratn, tn.size()));
+    HIPCHK(h, dalloc(h, &h->d_rat, tab.size()));
+    HIPCHK(h, dalloc(h, &h->d_ratn, tn.size()));
     HIPCHK(h, hipMemcpy(h->d_rat, tab.data(), 8 * tab.size(), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_ratn, tn.data(), 4 * tn.size(), hipMemcpyHostToDevice));
   }
@@ -810,6 +845,9 @@ static int ensure_outputs(letkf_gpu_t* h) {
 
 int letkf_gpu_run(letkf_gpu_t* h) {
   if (!h) return LETKF_EINVAL;
+  // whatever happens below, the previous analysis no longer matches the
+  // inputs: a run that fails leaves no analysis to fetch
+  h->have_run = false;
   if (!h->have_state || !h->have_obs || !h->have_loc || !h->have_solver || !h->have_vars)
     return fail(h, LETKF_ESTATE, "set_state, set_obs, set_statevars, set_localizer and "
                                  "set_solver must precede run");
@@ -820,9 +858,6 @@ int letkf_gpu_run(letkf_gpu_t* h) {
                 "has %d", h->loc.max_var_lev, h->loc.nlev);
   if (h->ns > 2048)
     return fail(h, LETKF_EUNSUPPORTED, "grid_ns %lld above 2048", (long long)h->ns);
-  // from here on the analysis buffers are overwritten: a run that fails
-  // leaves no analysis to fetch
-  h->have_run = false;
   const char* evj = getenv("LETKF_GPU_EIGEN");
   const bool mat = !(evj && strcmp(evj, "jacobi") == 0) && h->tune_mat &&
                    transform_mat_supported((int)h->k);
@@ -932,8 +967,8 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     if (total > b_list_cap) {
       // this set's previous chunk is finished (the search stream waited for it)
       const int64_t cap = total + total / 4 + 1024;
-      RUNCHK(dalloc(&b_list_idx, (size_t)cap));
-      RUNCHK(dalloc(&b_list_w, (size_t)cap));
+      RUNCHK(dalloc(h, &b_list_idx, (size_t)cap));
+      RUNCHK(dalloc(h, &b_list_w, (size_t)cap));
       b_list_cap = cap;
       C.list_idx = b_list_idx;
       C.list_w = b_list_w;
@@ -970,9 +1005,10 @@ int letkf_gpu_run(letkf_gpu_t* h) {
                             h->tune_trireg != 0);
       RUNCHK(hipGetLastError());
       // problems the QL could not finish go to the Jacobi solver
-      launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_status, h->d_ana);
+      RUNCHK(launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_status,
+                                    h->d_ana));
     } else {
-      launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, nullptr, h->d_ana);
+      RUNCHK(launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, nullptr, h->d_ana));
     }
     RUNCHK(hipGetLastError());
     RUNCHK(hipEventRecord(ce[5], st));
@@ -1044,13 +1080,13 @@ int letkf_gpu_get_state_members(letkf_gpu_t* h, float* members, float* mean, flo
   if (nij * ns <= 0) return LETKF_OK;
   float *dm = nullptr, *dmu = nullptr, *dsp = nullptr;
   auto done = [&](hipError_t e) {
-    dfree(dm); dfree(dmu); dfree(dsp);
+    dfree(h, dm); dfree(h, dmu); dfree(h, dsp);
     return e;
   };
   hipError_t e = hipSuccess;
-  if (members && e == hipSuccess) e = dalloc(&dm, (size_t)(k * ns * nij));
-  if (mean && e == hipSuccess) e = dalloc(&dmu, (size_t)(ns * nij));
-  if (sprd && e == hipSuccess) e = dalloc(&dsp, (size_t)(ns * nij));
+  if (members && e == hipSuccess) e = dalloc(h, &dm, (size_t)(k * ns * nij));
+  if (mean && e == hipSuccess) e = dalloc(h, &dmu, (size_t)(ns * nij));
+  if (sprd && e == hipSuccess) e = dalloc(h, &dsp, (size_t)(ns * nij));
   if (e == hipSuccess) {
     launch_members_from_state(h->stream, nij, ns, (int)k, h->d_ana, h->d_amean, h->d_asprd, dm,
                               dmu, dsp);
@@ -1071,17 +1107,30 @@ int letkf_gpu_get_state(letkf_gpu_t* h, float* state, float* mean, float* sprd) 
   if (!h) return LETKF_EINVAL;
   if (!h->have_run) return fail(h, LETKF_ESTATE, "no analysis: call letkf_gpu_run first");
   const int64_t k = h->k, ns = h->ns, nij = h->nij;
-  std::vector<double> buf;
-  auto conv = [&](double* src, float* dst, int64_t n) -> int {
-    buf.resize((size_t)n);
-    HIPCHK(h, hipMemcpy(buf.data(), src, 8 * n, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < n; ++i) dst[i] = (float)buf[i];
-    return LETKF_OK;
+  const int64_t n_max = std::max(k * ns * nij, ns * nij);
+  if (n_max <= 0) return LETKF_OK;
+  // converted on the device, then copied down in slices of at most 16M
+  // values through one staging buffer (host memory holds only the caller's
+  // FP32 arrays)
+  const int64_t slice = std::min<int64_t>(n_max, 1ll << 24);
+  float* stage = nullptr;
+  HIPCHK(h, dalloc(h, &stage, (size_t)slice));
+  hipError_t e = hipSuccess;
+  auto conv = [&](const double* src, float* dst, int64_t n) {
+    for (int64_t i = 0; i < n && e == hipSuccess; i += slice) {
+      const int64_t c = std::min(slice, n - i);
+      launch_f64_to_f32(h->stream, c, src + i, stage);
+      e = hipGetLastError();
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(dst + i, stage, 4 * c, hipMemcpyDeviceToHost, h->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
   };
-  int rc;
-  if (state && (rc = conv(h->d_ana, state, k * ns * nij))) return rc;
-  if (mean && (rc = conv(h->d_amean, mean, ns * nij))) return rc;
-  if (sprd && (rc = conv(h->d_asprd, sprd, ns * nij))) return rc;
+  if (state) conv(h->d_ana, state, k * ns * nij);
+  if (mean) conv(h->d_amean, mean, ns * nij);
+  if (sprd) conv(h->d_asprd, sprd, ns * nij);
+  dfree(h, stage);
+  HIPCHK(h, e);
   return LETKF_OK;
 }
 
@@ -1149,8 +1198,8 @@ int letkf_gpu_get_selection(letkf_gpu_t* h, int64_t ij, int32_t group, int32_t* 
   const int64_t total = h->h_tot[2] + PadCount()(h->h_tot[3]) + kListAlign;
   if (total > h->list_cap) {
     const int64_t c2 = total + 1024;
-    HIPCHK(h, dalloc(&h->d_list_idx, (size_t)c2));
-    HIPCHK(h, dalloc(&h->d_list_w, (size_t)c2));
+    HIPCHK(h, dalloc(h, &h->d_list_idx, (size_t)c2));
+    HIPCHK(h, dalloc(h, &h->d_list_w, (size_t)c2));
     h->list_cap = c2;
     C.list_idx = h->d_list_idx;
     C.list_w = h->d_list_w;

@@ -161,6 +161,7 @@ struct ObsFields {
   int32_t* i[kI];
   float* hx;
 };
+void launch_f64_to_f32(hipStream_t st, int64_t n, const double* in, float* out);
 void launch_state_from_members(hipStream_t st, int64_t nij, int64_t ns, int k,
                                const float* members, float* state, float* mean, float* sprd);
 void launch_members_from_state(hipStream_t st, int64_t nij, int64_t ns, int k,
@@ -177,10 +178,10 @@ void launch_search(hipStream_t st, bool fill, const LocDev& L,
                    const ChunkDev& C, const DiagDev& D);
 void launch_product_f64(hipStream_t st, const ObsDev& O, const StateDev& S,
                         const ChunkDev& C, double* B, double* bvec);
-void launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
-                            const VarsDev& V, const ChunkDev& C,
-                            const DiagDev& D, const double* B,
-                            const double* bvec, const int* only_flagged, double* ana);
+hipError_t launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDev& S,
+                                  const VarsDev& V, const ChunkDev& C, const DiagDev& D,
+                                  const double* B, const double* bvec, const int* only_flagged,
+                                  double* ana);
 void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev& S,
                        const ChunkDev& C, const int32_t* perm, double* B, double* bvec);
 void launch_bucket_keys(hipStream_t st, int mode, int bbits, int64_t np, int G, int64_t i0,

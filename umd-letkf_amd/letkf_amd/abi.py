@@ -9,6 +9,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
+
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libletkf_gpu.so")
 
@@ -159,8 +161,19 @@ def load_library(path: str | None = None) -> C.CDLL:
     return lib
 
 
+_NP_OF = {C.c_float: np.float32, C.c_double: np.float64, C.c_int32: np.int32,
+          C.c_int64: np.int64, C.c_uint64: np.uint64}
+
+
 def ptr(a, ctype):
-    """Pointer to a contiguous numpy array (None -> NULL)."""
+    """Pointer to a contiguous numpy array (None -> NULL).  The array's
+    dtype must be the C type's, and it must be contiguous (column-major for
+    the reference's multi-dimensional arrays is the caller's contract)."""
     if a is None:
         return None
+    want = _NP_OF.get(ctype)
+    if want is not None and a.dtype != want:
+        raise TypeError(f"array of dtype {a.dtype} passed as {ctype.__name__}")
+    if not (a.flags.f_contiguous or a.flags.c_contiguous):
+        raise ValueError("non-contiguous array passed to the C ABI")
     return a.ctypes.data_as(C.POINTER(ctype))

@@ -145,18 +145,30 @@ class Problem:
         return int(self.var_s.size)
 
     def validate(self):
-        k, ns, n = self.k, self.ns, self.nij
-        assert self.state.shape == (k, ns, n) and self.state.dtype == np.float32
-        assert self.mean.shape == (ns, n) and self.sprd.shape == (ns, n)
-        for a in (self.lat, self.lon):
-            assert a.shape == (n,) and a.dtype == np.float32
-        assert self.mask.shape == (n,) and self.mask.dtype == np.int32
-        no = self.nobs
-        for a in (self.olon, self.ozdim, self.otime, self.oval, self.oerr, self.hxmean):
-            assert a.shape == (no,) and a.dtype == np.float32
-        assert self.hx.shape == (k, no) and self.hx.dtype == np.float32
-        assert self.obsid.dtype == np.int32 and self.platid.dtype == np.int32
-        assert self.bounds.shape == (2, self.nvars)
+        """Check every array against the layout the C ABI reads through raw
+        pointers: shape, dtype, and Fortran (column-major) contiguity.  A
+        C-ordered or wrongly typed array would otherwise be read as
+        something else; raises ValueError naming the field."""
+        k, ns, n, no, nv = self.k, self.ns, self.nij, self.nobs, self.nvars
+        f32, i32 = np.float32, np.int32
+        spec = [("state", (k, ns, n), f32), ("mean", (ns, n), f32), ("sprd", (ns, n), f32),
+                ("lat", (n,), f32), ("lon", (n,), f32), ("mask", (n,), i32),
+                ("olat", (no,), f32), ("olon", (no,), f32), ("ozdim", (no,), f32),
+                ("otime", (no,), f32), ("oval", (no,), f32), ("oerr", (no,), f32),
+                ("obsid", (no,), i32), ("platid", (no,), i32), ("hx", (k, no), f32),
+                ("hxmean", (no,), f32), ("var_s", (nv,), i32), ("var_lev", (nv,), i32),
+                ("inc_max", (nv,), f32), ("bounds", (2, nv), f32)]
+        for name, shape, dt in spec:
+            a = getattr(self, name)
+            if not isinstance(a, np.ndarray):
+                raise ValueError(f"Problem.{name}: not a numpy array")
+            if a.shape != shape:
+                raise ValueError(f"Problem.{name}: shape {a.shape}, expected {shape}")
+            if a.dtype != dt:
+                raise ValueError(f"Problem.{name}: dtype {a.dtype}, expected {np.dtype(dt)}")
+            if not a.flags.f_contiguous:
+                raise ValueError(f"Problem.{name}: not Fortran-contiguous (column-major, "
+                                 "as the reference's arrays)")
 
     def subset_obs(self, sel):
         """New problem with only the observations ``sel`` (index array)."""
