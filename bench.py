@@ -63,10 +63,20 @@ def product_kernel(path, precision):
     return "k_product_f64" if path == "plain" else "k_product_mx"
 
 
-def roofline(t, k, path, precision, workload):
+def tuning_key(tun):
+    return "order{order}_bb{bucket_bits}".format(**tun) if tun else "default"
+
+
+def roofline(t, k, path, precision, workload, tun=None, slabs_per_problem=1.0, ns=1,
+             n_gp=0, n_obs=0):
     """Roofline of the product kernel (the largest single kernel): the
     algorithmic flops F_prod = sum over problems of 2k^2 m + 2k m per launch
-    over the kernel's own HIP-event time on the library stream."""
+    over the kernel's own HIP-event time on the library stream.  Also the
+    whole analysis: T_roof = max(F_prod / P_prod + F_eig / P_fp64 +
+    F_apply / P_apply, B_comp / BW) over the measured analysis time, with
+    F_eig = 11k^3 + 4k^2 per problem (symmetric QR with vectors, the
+    transform W, w), F_apply = 2k^2 per slab of each problem and the
+    compulsory bytes B_comp = N_gp (8 ns k + 16 ns + 12) + N_obs (k+3) 4."""
     flop = (2.0 * k * k + 2.0 * k) * t["n_pairs"]
     ms = t["product_ms"]
     nl = max(int(t["n_launch"]), 1)
@@ -79,22 +89,41 @@ def roofline(t, k, path, precision, workload):
                 launches=nl, avg_launch_ms=round(ms / nl, 4),
                 algorithmic=f"{flop / nl:.4e} flop and {gather / nl:.4e} gathered bytes per launch",
                 gather_GBps=round(gather / (ms * 1e-3) / 1e9, 1) if ms > 0 else 0.0)
+    # the whole analysis
+    npb = float(t["n_problems"])
+    f_eig = npb * (11.0 * k ** 3 + 4.0 * k * k)
+    f_apply = npb * 2.0 * k * k * slabs_per_problem
+    apply_peak = PEAK["fp64"] if (path == "plain" or k < 64) else peak
+    b_comp = n_gp * (8.0 * ns * k + 16.0 * ns + 12.0) + n_obs * (k + 3) * 4.0
+    t_cmp = flop / (peak * 1e12) + f_eig / (PEAK["fp64"] * 1e12) + f_apply / (apply_peak * 1e12)
+    t_mem = b_comp / (HBM_GBS * 1e9)
+    t_roof = max(t_cmp, t_mem)
+    if t.get("total_ms", 0) > 0:
+        roof["path_frac"] = round(t_roof / (t["total_ms"] * 1e-3), 5)
+        roof["path"] = {"t_roof_ms": round(1e3 * t_roof, 3), "t_measured_ms": round(t["total_ms"], 3),
+                        "F_prod": f"{flop:.4e}", "F_eig": f"{f_eig:.4e}",
+                        "F_apply": f"{f_apply:.4e}", "B_comp": f"{b_comp:.4e}",
+                        "bound": "compute" if t_cmp >= t_mem else "hbm",
+                        "peaks_TFLOPs": {"prod": peak, "eig": PEAK["fp64"], "apply": apply_peak}}
     try:
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        ent = tr.get(f"{workload}/{path}/{precision}")
+        key = f"{workload}/{path}/{precision}/{tuning_key(tun)}"
+        ent = tr.get(key)
         if ent and ent.get("kernel") == kern:
             roof["traffic"] = ent["bytes_per_dispatch"]
             roof["traffic_note"] = (f"HBM bytes per dispatch from rocprofv3 --pmc FETCH_SIZE x "
                                     f"{FETCH_SCALE} + WRITE_SIZE x {WRITE_SCALE} "
                                     f"({ent['source']}); launches of {ent['problems_per_dispatch']} "
-                                    f"problems")
+                                    f"problems under the same launch choice ({tuning_key(tun)})")
+        else:
+            roof["traffic_note"] = f"no counter record for {key}"
     except (OSError, ValueError):
         pass
     return roof
 
 
-def summarize_pmc(fetch_csv, write_csv, workload, path, precision, problems):
+def summarize_pmc(fetch_csv, write_csv, workload, path, precision, problems, tun=None):
     """Per-dispatch HBM traffic of the product kernel from two rocprofv3
     --pmc runs, merged into profiles/traffic.json."""
     import csv
@@ -118,48 +147,88 @@ def summarize_pmc(fetch_csv, write_csv, workload, path, precision, problems):
             tr = json.load(f)
     except (OSError, ValueError):
         tr = {}
-    tr[f"{workload}/{path}/{precision}"] = ent
+    tr[f"{workload}/{path}/{precision}/{tuning_key(tun)}"] = ent
     with open(TRAFFIC_FILE, "w") as f:
         json.dump(tr, f, indent=1, sort_keys=True)
     print(json.dumps(ent))
 
 
+def host_cpu():
+    """Host cores as the OS reports them: nproc (online CPUs), the CPUs this
+    process may run on, and the model name."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity": aff, "model": model}
+
+
+def cpu_threads():
+    """Worker threads of the CPU legs: every CPU this process may use, at
+    most 16 (one GPU's share of the host on the GPU machines)."""
+    return max(1, min(16, host_cpu()["affinity"] or 1))
+
+
 def cpu_baseline(prob, solver, seconds, nthreads):
-    """The FP64 oracle on a bounded random sample of unmasked gridpoints,
-    one worker per host thread with the reference's round-robin gridpoint
-    ownership: a pilot run sizes the sample (and warms up), then the best
-    of three runs over it is reported, with the share of the reference's
-    timer phases (obs_search, letkf_core, letkf_trans).  The oracle's
-    search-structure set-up is timed apart and left out, like the GPU's
-    set_obs is left out of the GPU's timed region."""
+    """The reference's CPU path: its gridpoint loop in FP32 with the
+    reference's order of operations in letkf_core_solve (sgemm products,
+    a symmetric eigensolve, the Pa hdxb_rinv product, the per-slab apply,
+    src/letkf_core.F90:92-141, src/letkf_solver.F90:203-379), on a bounded
+    random sample of unmasked gridpoints, one worker per host thread with
+    the reference's round-robin gridpoint ownership.  A pilot run sizes the
+    sample (and warms up), then the best of three runs over it is reported
+    with the share of the reference's timer phases (obs_search,
+    letkf_core, letkf_trans).  The search-structure set-up is timed apart
+    and left out, like the GPU's set_obs.  The FP64 oracle then analyses
+    the same sample once: the GPU's error (and the FP32 path's own) is
+    measured against it."""
     import oracle as O
     g = np.random.default_rng(1234)
     pool = np.nonzero(prob.mask == 0)[0]
     pilot = np.sort(g.choice(pool, min(256, pool.size), replace=False))
-    r = O.analysis(prob, points=pilot, nthreads=nthreads)
+    r = O.analysis(prob, points=pilot, nthreads=nthreads, f32=True)
     per = r["loop_s"] / pilot.size
     n = int(min(pool.size, max(256, seconds / 3.0 / max(per, 1e-7))))
     pts = np.sort(g.choice(pool, n, replace=False))
-    runs = [O.analysis(prob, points=pts, nthreads=nthreads) for _ in range(3)]
-    orc = min(runs, key=lambda x: x["loop_s"])
-    ph = np.array([orc["obs_search_s"], orc["letkf_core_s"], orc["letkf_trans_s"]])
+    runs = [O.analysis(prob, points=pts, nthreads=nthreads, f32=True) for _ in range(3)]
+    ref32 = min(runs, key=lambda x: x["loop_s"])
+    ph = np.array([ref32["obs_search_s"], ref32["letkf_core_s"], ref32["letkf_trans_s"]])
     ph = ph / max(ph.sum(), 1e-12)
-    base = {"value": round(n / orc["loop_s"], 2), "unit": "gridpoints/s", "cores": nthreads,
-            "kind": "port",
-            "sample": f"{n} random unmasked gridpoints of the same analysis, FP64 oracle, "
-                      f"best of 3 gridpoint loops {orc['loop_s']:.2f} s (after a warm-up; "
-                      f"+ {orc['setup_s']:.1f} s search set-up, not counted)",
-            "phase_share": {"obs_search": round(float(ph[0]), 3),
-                            "letkf_core": round(float(ph[1]), 3),
-                            "letkf_trans": round(float(ph[2]), 3)}}
-    st, mn, sp = solver.state_f64()
-    d = solver.diag()
-    sel_ok = bool(np.array_equal(d["lg_selhash"][pts, :].T, orc["lg_hash"]) and
-                  np.array_equal(d["lg_obscount"][pts, :].T.astype(np.int64), orc["lg_m"]))
+    hc = host_cpu()
+    orc = O.analysis(prob, points=pts, nthreads=nthreads)
 
     def rel(a, b):
         den = np.abs(b).max()
         return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
+    base = {"value": round(n / ref32["loop_s"], 2), "unit": "gridpoints/s", "cores": nthreads,
+            "kind": "port",
+            "sample": f"{n} random unmasked gridpoints of the same analysis; the reference's "
+                      f"loop in FP32 in its order of operations (letkf_core_solve: sgemm, "
+                      f"FP32 Householder + QL eigensolve, Pa hdxb_rinv; per-slab apply), "
+                      f"best of 3 gridpoint loops {ref32['loop_s']:.2f} s after a warm-up "
+                      f"(+ {ref32['setup_s']:.1f} s search set-up, not counted)",
+            "host": hc,
+            "threads_note": f"{nthreads} worker threads: the CPUs this process may use "
+                            f"({hc['affinity']} of nproc {hc['nproc']}), at most 16",
+            "phase_share": {"obs_search": round(float(ph[0]), 3),
+                            "letkf_core": round(float(ph[1]), 3),
+                            "letkf_trans": round(float(ph[2]), 3)},
+            "fp64_oracle_gridpoints_per_s": round(n / orc["loop_s"], 2),
+            "fp32_reference_error_vs_fp64_oracle": {
+                "mean_rel": rel(ref32["mean"], orc["mean"]),
+                "sprd_rel": rel(ref32["sprd"], orc["sprd"]),
+                "perturbations_rel": rel(ref32["state"] - ref32["mean"][None],
+                                         orc["state"] - orc["mean"][None])}}
+    st, mn, sp = solver.state_f64()
+    d = solver.diag()
+    sel_ok = bool(np.array_equal(d["lg_selhash"][pts, :].T, orc["lg_hash"]) and
+                  np.array_equal(d["lg_obscount"][pts, :].T.astype(np.int64), orc["lg_m"]))
     inc_o = orc["mean"] - prob.mean[:, pts]
     pert_o = orc["state"] - orc["mean"][None]
     err = {"sample_gridpoints": n, "selection_bit_exact": sel_ok,
@@ -187,7 +256,7 @@ def bench_cycles(args):
     from letkf_amd import synth
     from letkf_amd.solver import LetkfGpuSolver
 
-    nt = max(1, min(16, os.cpu_count() or 1))
+    nt = cpu_threads()
     t0 = time.time()
     prob = synth.config3()
     mem = np.asfortranarray((prob.state + prob.mean[None]).transpose(2, 1, 0), np.float32)
@@ -293,7 +362,7 @@ def bench_c4(args):
     from letkf_amd import synth
     from letkf_amd.solver import LetkfGpuSolver
 
-    nt = max(1, min(16, os.cpu_count() or 1))
+    nt = cpu_threads()
     t0 = time.time()
     prob = synth.config4()
     log(f"[c4] inputs: nij={prob.nij} ns={prob.ns} k={prob.k} nobs={prob.nobs} "
@@ -353,7 +422,8 @@ def bench_c4(args):
                                   "(13,000 profiles x 75 + 25,000 surface), LOC_NOVRT 700->200 km "
                                   "cut-off, RTPS 0.6", "n_gp": prob.nij, "ns": prob.ns,
                       "k": prob.k, "n_obs": prob.nobs, "parallelism": "single GPU"},
-           "roofline": roofline(tm, prob.k, "batched", args.precision, "c4"),
+           "roofline": roofline(tm, prob.k, "batched", args.precision, "c4", None, prob.ns,
+                                prob.ns, prob.nij, prob.nobs),
            "batched": main_res, "oracle_sample": int(pts.size)}
     if args.no_fp64_compare:
         # the batched run alone (profiling)
@@ -393,14 +463,23 @@ def main():
     ap.add_argument("--problems-per-dispatch", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch choices instead of letkf_gpu_autotune")
+    ap.add_argument("--tuning", default=None,
+                    help="launch choices order,bucket_bits[,tridiag_reg,overlap,chunk_log2] set "
+                         "instead of autotuning (as a saved autotune result is reused)")
     ap.add_argument("--cycles", type=int, default=10, help="c5: number of analysis cycles")
     ap.add_argument("--cycle-sample", type=int, default=1000,
                     help="c5: gridpoints per cycle checked against the FP64 oracle; "
                          "c4: gridpoints checked")
     args = ap.parse_args()
+    tun_fixed = None
+    if args.tuning:
+        v = [int(x) for x in args.tuning.split(",")]
+        v += [1, 0, 20][len(v) - 2:] if len(v) < 5 else []
+        tun_fixed = dict(order=v[0], bucket_bits=v[1], tridiag_reg=v[2], overlap=v[3],
+                         chunk_log2=v[4])
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, args.config, args.path, args.precision,
-                      args.problems_per_dispatch)
+                      args.problems_per_dispatch, tun_fixed)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; a multi-GPU "
@@ -425,9 +504,15 @@ def main():
     upload_s = time.perf_counter() - tu
     log(f"[bench] inputs uploaded ({upload_s:.2f} s)")
     tuning = None
-    if not args.no_autotune:
+    if tun_fixed:
+        solver.set_tuning(**tun_fixed)
+        tuning = solver.tuning()
+        log(f"[bench] tuning set: {tuning}")
+    elif not args.no_autotune:
         tuning = solver.autotune(repeats=2)
         log(f"[bench] autotune: {tuning}")
+    else:
+        tuning = solver.tuning()
     for i in range(args.warmup):
         solver.run()
         log(f"[bench] warmup {i} done")
@@ -446,7 +531,10 @@ def main():
     tavg = {kk: v / args.steps for kk, v in ph_acc.items()}
     ms = 1e3 * elapsed / args.steps
     value = prob.nij * args.steps / elapsed
-    roof = roofline(tavg, prob.k, args.path, args.precision, args.config)
+    G = prob.loc.maxgroups()
+    slabs = prob.ns / G if G > 1 else prob.ns     # slabs per (gridpoint, group) problem
+    roof = roofline(tavg, prob.k, args.path, args.precision, args.config, tuning, slabs,
+                    prob.ns, prob.nij, prob.nobs)
 
     out = {
         "metric": "analysis gridpoints/sec at k=50, batched mixed-prec vs FP64 GPU; rel err vs FP64",
@@ -472,7 +560,7 @@ def main():
     }
 
     if not args.no_cpu:
-        nt = max(1, min(16, os.cpu_count() or 1))
+        nt = cpu_threads()
         out["cpu_baseline"], out["error_vs_fp64_oracle"] = cpu_baseline(
             prob, solver, args.cpu_seconds, nt)
     solver.final()

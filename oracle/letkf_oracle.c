@@ -767,6 +767,160 @@ static int analyse_point(const orc_ctx* c, orc_scratch* s, int64_t ij,
   return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* The reference's loop in REAL(4), for the CPU baseline: the same search,  */
+/* groups and localization as above (the reference computes those in REAL  */
+/* with FP64 GC internals), then letkf_core_solve and the apply and        */
+/* post-processing of src/letkf_solver.F90:224-379 in FP32.                 */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  float *hdxb, *rdiag, *rloc, *dep, *work, *trans, *bkg, *ana, *mean;
+  int64_t cap_m;
+} r32_scratch;
+
+static int r32_reserve(r32_scratch* s, int64_t k, int64_t m) {
+  if (m <= s->cap_m) return 0;
+  const int64_t cap = m + m / 2 + 64;
+  free(s->hdxb); free(s->rdiag); free(s->rloc); free(s->dep); free(s->work);
+  s->hdxb = (float*)malloc(sizeof(float) * k * cap);
+  s->rdiag = (float*)malloc(sizeof(float) * cap);
+  s->rloc = (float*)malloc(sizeof(float) * cap);
+  s->dep = (float*)malloc(sizeof(float) * cap);
+  s->work = (float*)malloc(sizeof(float) * r32_core_work((int)k, (int)cap));
+  s->cap_m = cap;
+  return (s->hdxb && s->rdiag && s->rloc && s->dep && s->work) ? 0 : -1;
+}
+
+static int analyse_point_f32(const orc_ctx* c, orc_scratch* s, r32_scratch* r, int64_t ij,
+                             int64_t slot, const orc_output* out) {
+  const orc_problem* p = c->p;
+  const int64_t k = p->k, ns = p->ns;
+  const int G = c->loc.maxgroups;
+  const float* st = p->state + k * ns * ij;
+  double* ost = out->state + k * ns * slot;
+  double* omean = out->mean + ns * slot;
+  double* osprd = out->sprd + ns * slot;
+  for (int g = 0; g < G; ++g)
+    if (out->lg_m) out->lg_m[g + (int64_t)G * slot] = 0;
+  if (p->mask[ij]) {                                          /* :207 */
+    for (int64_t i = 0; i < k * ns; ++i) ost[i] = (double)st[i];
+    for (int64_t i = 0; i < ns; ++i) {
+      omean[i] = (double)p->mean[ns * ij + i];
+      osprd[i] = (double)p->sprd[ns * ij + i];
+    }
+    return 0;
+  }
+  double tp = now_s();
+  const float lat_ij = p->lat[ij];
+  const float rs = loc_maxhz(&c->loc, lat_ij) * ORC_STDEV2MAX;
+  double g3[3];
+  orc_xyz(p->lat[ij], p->lon[ij], g3);
+  const int64_t n = search(c, g3, orc_thr2(rs), s->cidx, s->cdist);
+  { const double t = now_s(); s->t_search += t - tp; tp = t; }
+  const int ngr = loc_groups(&c->loc, p, ij, s->gid, s->slab_group);
+  float* bkg = r->bkg;
+  float* x = r->ana;
+  float* mean = r->mean;
+  for (int64_t i = 0; i < k * ns; ++i) { bkg[i] = st[i]; x[i] = st[i]; }
+  const float* bmean = p->mean + ns * ij;
+  const float* bsprd = p->sprd + ns * ij;
+  for (int lg = 0; lg < ngr; ++lg) {                          /* :241-292 */
+    double tq = now_s();
+    s->t_trans += tq - tp;
+    const int64_t m = localize_group(c, lat_ij, s->gid[lg], n, s->cidx, s->cdist, s->kidx,
+                                     s->krloc);
+    tp = now_s();
+    s->t_search += tp - tq;
+    if (out->lg_m) out->lg_m[lg + (int64_t)G * slot] = m;
+    if (m == 0) continue;
+    if (r32_reserve(r, k, m)) return -1;
+    for (int64_t i = 0; i < m; ++i) {                          /* :224-229, 251-262 */
+      const int64_t o = s->kidx[i];
+      memcpy(r->hdxb + k * i, p->hx + k * o, sizeof(float) * k);
+      r->rdiag[i] = p->oerr[o];
+      r->dep[i] = p->oval[o] - p->hxmean[o];
+      r->rloc[i] = (float)s->krloc[i];
+    }
+    if (r32_core_solve((int)k, (int)m, r->hdxb, r->rdiag, r->rloc, r->dep, 1.0f, r->trans,
+                       r->work)) return -1;
+    tq = now_s();
+    s->t_core += tq - tp;
+    tp = tq;
+    for (int64_t sl = 0; sl < ns; ++sl) {                      /* :285-289, sgemv */
+      if (s->slab_group[sl] != lg) continue;
+      const float* xb = bkg + k * sl;
+      float* xa = x + k * sl;
+      for (int64_t j = 0; j < k; ++j) {
+        float acc = 0.0f;
+        for (int64_t i = 0; i < k; ++i) acc += xb[i] * r->trans[i + k * j];
+        xa[j] = acc;
+      }
+    }
+  }
+  const float fk = (float)k;
+  for (int64_t sl = 0; sl < ns; ++sl) {                        /* :299-303 */
+    float* xs = x + k * sl;
+    float sum = 0.0f;
+    for (int64_t j = 0; j < k; ++j) xs[j] = xs[j] + bmean[sl];
+    for (int64_t j = 0; j < k; ++j) sum += xs[j];
+    mean[sl] = sum / fk;
+    for (int64_t j = 0; j < k; ++j) xs[j] = xs[j] - mean[sl];
+  }
+  for (int v = 0; v < p->nvars; ++v) {                         /* :308-318 */
+    const float inc = p->inc_max[v], blo = p->bounds[2 * v], bhi = p->bounds[2 * v + 1];
+    for (int j = p->var_s[v]; j < p->var_s[v] + p->var_lev[v]; ++j) {
+      float rr = mean[j] - bmean[j];
+      rr = fmaxf(rr, -inc);
+      rr = fminf(rr, inc);
+      rr = fmaxf(rr, blo - bmean[j]);
+      rr = fminf(rr, bhi - bmean[j]);
+      mean[j] = rr + bmean[j];
+    }
+  }
+  if (p->rtps > 0.0f) {                                        /* :322-335 */
+    for (int64_t sl = 0; sl < ns; ++sl) {
+      float* xs = x + k * sl;
+      float ss = 0.0f;
+      for (int64_t j = 0; j < k; ++j) ss += xs[j] * xs[j];
+      const float sa = sqrtf(ss / (float)(k - 1));
+      if (sa <= 0.0f) continue;
+      const float f = p->rtps * (bsprd[sl] - sa) / sa + 1.0f;
+      for (int64_t j = 0; j < k; ++j) xs[j] *= f;
+    }
+  }
+  if (p->rtpp > 0.0f)                                          /* :339-343 */
+    for (int64_t i = 0; i < k * ns; ++i) x[i] = p->rtpp * bkg[i] + (1.0f - p->rtpp) * x[i];
+  if (p->mul != 1.0f)                                          /* :347-349 */
+    for (int64_t i = 0; i < k * ns; ++i) x[i] *= p->mul;
+  for (int v = 0; v < p->nvars; ++v) {                         /* :357-371 */
+    const float blo = p->bounds[2 * v], bhi = p->bounds[2 * v + 1];
+    for (int j = p->var_s[v]; j < p->var_s[v] + p->var_lev[v]; ++j) {
+      float* xs = x + k * j;
+      float sum = 0.0f;
+      for (int64_t mb = 0; mb < k; ++mb) {
+        float rr = xs[mb];
+        rr = fmaxf(rr, blo - mean[j]);
+        rr = fminf(rr, bhi - mean[j]);
+        xs[mb] = rr;
+      }
+      for (int64_t mb = 0; mb < k; ++mb) sum += xs[mb];
+      const float rr = sum / fk;
+      for (int64_t mb = 0; mb < k; ++mb) xs[mb] -= rr;
+      mean[j] += rr;
+    }
+  }
+  for (int64_t sl = 0; sl < ns; ++sl) {                        /* :375-379 */
+    const float* xs = x + k * sl;
+    float ss = 0.0f;
+    for (int64_t j = 0; j < k; ++j) ss += xs[j] * xs[j];
+    osprd[sl] = (double)sqrtf(ss / (float)(k - 1));
+    omean[sl] = (double)mean[sl];
+    for (int64_t j = 0; j < k; ++j) ost[j + k * sl] = (double)(xs[j] + mean[sl]);
+  }
+  s->t_trans += now_s() - tp;
+  return 0;
+}
+
 typedef struct {
   const orc_ctx* c;
   const int64_t* pts;
@@ -775,26 +929,41 @@ typedef struct {
   const orc_output* out;
   int status;
   double t_search, t_core, t_trans;
+  int f32;
 } worker_arg;
 
 static void* worker(void* argp) {
   worker_arg* a = (worker_arg*)argp;
   orc_scratch s;
-  if (scratch_init(&s, a->c->p, a->c->loc.maxgroups)) { a->status = -1; scratch_free(&s); return NULL; }
+  r32_scratch r;
+  memset(&r, 0, sizeof(r));
+  const orc_problem* p = a->c->p;
+  if (scratch_init(&s, p, a->c->loc.maxgroups)) { a->status = -1; scratch_free(&s); return NULL; }
+  if (a->f32) {
+    r.trans = (float*)malloc(sizeof(float) * p->k * p->k);
+    r.bkg = (float*)malloc(sizeof(float) * p->k * p->ns);
+    r.ana = (float*)malloc(sizeof(float) * p->k * p->ns);
+    r.mean = (float*)malloc(sizeof(float) * p->ns);
+    if (!r.trans || !r.bkg || !r.ana || !r.mean) a->status = -1;
+  }
   /* round-robin ownership like letkf_mpi_setgrid (src/letkf_mpi.F90:213-225) */
-  for (int64_t i = a->tid; i < a->npts; i += a->nthreads) {
-    if (analyse_point(a->c, &s, a->pts[i], i, a->out)) { a->status = -1; break; }
+  for (int64_t i = a->tid; i < a->npts && !a->status; i += a->nthreads) {
+    const int rc = a->f32 ? analyse_point_f32(a->c, &s, &r, a->pts[i], i, a->out)
+                          : analyse_point(a->c, &s, a->pts[i], i, a->out);
+    if (rc) { a->status = -1; break; }
   }
   a->t_search = s.t_search;
   a->t_core = s.t_core;
   a->t_trans = s.t_trans;
   scratch_free(&s);
+  free(r.hdxb); free(r.rdiag); free(r.rloc); free(r.dep); free(r.work);
+  free(r.trans); free(r.bkg); free(r.ana); free(r.mean);
   return NULL;
 }
 
 
-int orc_analysis(const orc_problem* p, const int64_t* pts, int64_t npts,
-                 int nthreads, const orc_output* out) {
+static int run_analysis(const orc_problem* p, const int64_t* pts, int64_t npts,
+                        int nthreads, const orc_output* out, int f32) {
   orc_ctx c;
   const double t0 = now_s();
   if (ctx_init(&c, p)) { ctx_free(&c); return -1; }
@@ -805,7 +974,7 @@ int orc_analysis(const orc_problem* p, const int64_t* pts, int64_t npts,
   worker_arg args[256];
   int status = 0;
   for (int t = 0; t < nthreads; ++t) {
-    args[t] = (worker_arg){&c, pts, npts, t, nthreads, out, 0, 0.0, 0.0, 0.0};
+    args[t] = (worker_arg){&c, pts, npts, t, nthreads, out, 0, 0.0, 0.0, 0.0, f32};
     if (nthreads == 1) worker(&args[t]);
     else pthread_create(&th[t], NULL, worker, &args[t]);
   }
@@ -825,6 +994,18 @@ int orc_analysis(const orc_problem* p, const int64_t* pts, int64_t npts,
   }
   ctx_free(&c);
   return status;
+}
+
+int orc_analysis(const orc_problem* p, const int64_t* pts, int64_t npts,
+                 int nthreads, const orc_output* out) {
+  return run_analysis(p, pts, npts, nthreads, out, 0);
+}
+
+/* the reference's loop in FP32 (CPU baseline); outputs widened to FP64,
+ * only state, mean, sprd, lg_m and seconds are written */
+int orc_analysis_f32(const orc_problem* p, const int64_t* pts, int64_t npts,
+                     int nthreads, const orc_output* out) {
+  return run_analysis(p, pts, npts, nthreads, out, 1);
 }
 
 int64_t orc_selection(const orc_problem* p, int64_t ij, int32_t group,

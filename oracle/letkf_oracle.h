@@ -15,6 +15,7 @@
 #ifndef LETKF_ORACLE_H
 #define LETKF_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 #include "../include/letkf_gpu.h"
 
@@ -88,6 +89,15 @@ int orc_eigh(int n, double* a, double* w, double* v);
 
 int orc_analysis(const orc_problem* p, const int64_t* pts, int64_t npts,
                  int nthreads, const orc_output* out);
+
+int orc_analysis_f32(const orc_problem* p, const int64_t* pts, int64_t npts,
+                     int nthreads, const orc_output* out);
+
+/* FP32 core in the reference's order (ref32_core.c) */
+int r32_syev(int n, float* a, float* w, float* e);
+size_t r32_core_work(int k, int m);
+int r32_core_solve(int k, int m, const float* hdxb, const float* rdiag, const float* rloc,
+                   const float* dep, float infl, float* trans, float* work);
 
 int64_t orc_selection(const orc_problem* p, int64_t ij, int32_t group,
                       int32_t* idx_out, float* rloc_out, float* dist_out,

@@ -84,6 +84,15 @@ def lib():
         L.orc_analysis.restype = C.c_int
         L.orc_analysis.argtypes = [C.POINTER(OrcProblem), _LP, C.c_int64,
                                    C.c_int, C.POINTER(OrcOutput)]
+        L.orc_analysis_f32.restype = C.c_int
+        L.orc_analysis_f32.argtypes = [C.POINTER(OrcProblem), _LP, C.c_int64,
+                                       C.c_int, C.POINTER(OrcOutput)]
+        L.r32_syev.restype = C.c_int
+        L.r32_syev.argtypes = [C.c_int, _FP, _FP, _FP]
+        L.r32_core_work.restype = C.c_size_t
+        L.r32_core_work.argtypes = [C.c_int, C.c_int]
+        L.r32_core_solve.restype = C.c_int
+        L.r32_core_solve.argtypes = [C.c_int, C.c_int, _FP, _FP, _FP, _FP, C.c_float, _FP, _FP]
         L.orc_selection.restype = C.c_int64
         L.orc_selection.argtypes = [C.POINTER(OrcProblem), C.c_int64,
                                     C.c_int32, _IP, _FP, _FP, C.c_int64]
@@ -146,6 +155,33 @@ def core_solve(hdxb, rdiag, rloc, dep, infl=1.0):
     return trans, ev
 
 
+def syev32(a):
+    """The FP32 eigensolver of the CPU baseline (ascending, vectors as
+    columns)."""
+    a = np.array(a, dtype=np.float32, order="F")
+    n = a.shape[0]
+    w = np.zeros(n, np.float32)
+    e = np.zeros(n, np.float32)
+    rc = lib().r32_syev(n, _p(a, C.c_float), _p(w, C.c_float), _p(e, C.c_float))
+    assert rc == 0
+    return w, a
+
+
+def core_solve32(hdxb, rdiag, rloc, dep, infl=1.0):
+    """letkf_core_solve in FP32, the reference's order (CPU baseline)."""
+    hdxb = np.asfortranarray(hdxb, dtype=np.float32)
+    k, m = hdxb.shape
+    f = lambda x: np.ascontiguousarray(x, np.float32)
+    rdiag, rloc, dep = f(rdiag), f(rloc), f(dep)
+    trans = np.zeros((k, k), np.float32, order="F")
+    work = np.zeros(lib().r32_core_work(k, m), np.float32)
+    rc = lib().r32_core_solve(k, m, _p(hdxb, C.c_float), _p(rdiag, C.c_float),
+                              _p(rloc, C.c_float), _p(dep, C.c_float), float(infl),
+                              _p(trans, C.c_float), _p(work, C.c_float))
+    assert rc == 0
+    return trans
+
+
 def eigh(a):
     a = np.array(a, dtype=np.float64, order="F")
     n = a.shape[0]
@@ -187,12 +223,14 @@ class _Bound:
         self.s = s
 
 
-def analysis(prob, points=None, nthreads=None):
+def analysis(prob, points=None, nthreads=None, f32=False):
     """Run the FP64 restated loop over ``points`` (default: all).
 
     Returns a dict with state (k, ns, npts), mean, sprd (ns, npts) in FP64
     and the diagnostics maxhz, obscount, lg_m, lg_obsloc, lg_hash
-    (maxgroups, npts).
+    (maxgroups, npts).  With f32=True the reference's loop runs in FP32
+    instead (the CPU baseline; results widened to FP64, diagnostics other
+    than lg_m not written).
     """
     prob.validate()
     b = _Bound(prob)
@@ -222,7 +260,8 @@ def analysis(prob, points=None, nthreads=None):
     secs = np.zeros(5)
     o.seconds = _p(secs, C.c_double)
     nt = nthreads or min(os.cpu_count() or 1, 16)
-    rc = lib().orc_analysis(C.byref(b.s), _p(pts, C.c_int64), n, nt, C.byref(o))
+    fn = lib().orc_analysis_f32 if f32 else lib().orc_analysis
+    rc = fn(C.byref(b.s), _p(pts, C.c_int64), n, nt, C.byref(o))
     if rc != 0:
         raise RuntimeError("oracle analysis failed")
     out["points"] = pts

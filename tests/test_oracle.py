@@ -597,3 +597,53 @@ def test_bounds_check_clip_binds():
     pa = out["state"][:, s2] - out["mean"][None, s2]
     pf = free["state"][:, s2] - free["mean"][None, s2]
     assert np.abs(pa - pf).max() < 1e-9
+
+
+# ------------------------------------------------------------ FP32 CPU baseline
+def test_syev32_against_numpy():
+    """The FP32 eigensolver of the CPU baseline (Householder + implicit QL)
+    against numpy's FP64 eigh: eigenvalues to FP32 precision relative to the
+    largest, orthonormal vectors, A reconstructed."""
+    g = np.random.default_rng(8)
+    for n in (2, 5, 20, 50, 100):
+        x = g.standard_normal((n, 3 * n))
+        a = x @ x.T + (n - 1) * np.eye(n)
+        w, v = O.syev32(a)
+        w0 = np.linalg.eigvalsh(a)
+        assert np.all(np.diff(w) >= 0)
+        assert np.abs(w - w0).max() <= 2e-6 * n * w0.max()
+        v = v.astype(np.float64)
+        assert np.abs(v.T @ v - np.eye(n)).max() < 1e-5 * n
+        assert np.abs(v @ np.diag(w.astype(np.float64)) @ v.T - a).max() <= 2e-6 * n * w0.max()
+
+
+@pytest.mark.parametrize("k,m", [(5, 1), (20, 7), (20, 40), (50, 300)])
+def test_core32_against_fp64_core(k, m):
+    """The FP32 core in the reference's order against the FP64 oracle core
+    (itself pinned against numpy and closed forms): FP32-level agreement."""
+    g = np.random.default_rng(k * 7 + m)
+    y = g.standard_normal((k, m))
+    y -= y.mean(0, keepdims=True)
+    rdiag = g.uniform(0.2, 1.0, m)
+    rloc = g.uniform(0.1, 1.0, m)
+    dep = g.standard_normal(m)
+    y32 = y.astype(np.float32)
+    t64, _ = O.core_solve(y32.astype(np.float64), np.float32(rdiag), np.float32(rloc),
+                          np.float32(dep))
+    t32 = O.core_solve32(y32, rdiag, rloc, dep)
+    assert np.abs(t32 - t64).max() <= 2e-5 * np.abs(t64).max()
+
+
+def test_analysis_f32_matches_fp64_oracle():
+    """The FP32 gridpoint loop selects the same observations as the FP64
+    oracle and agrees with it to FP32 precision (config 1)."""
+    p = synth.config1()
+    g = np.random.default_rng(3)
+    pts = np.sort(g.choice(np.nonzero(p.mask == 0)[0], 300, replace=False))
+    a = O.analysis(p, points=pts)
+    b = O.analysis(p, points=pts, f32=True)
+    assert np.array_equal(a["lg_m"], b["lg_m"])
+    assert np.abs(b["mean"] - a["mean"]).max() <= 1e-5 * np.abs(a["mean"]).max()
+    assert np.abs(b["sprd"] - a["sprd"]).max() <= 1e-4 * np.abs(a["sprd"]).max()
+    pa, pb = a["state"] - a["mean"][None], b["state"] - b["mean"][None]
+    assert np.abs(pb - pa).max() <= 1e-4 * np.abs(pa).max()
