@@ -242,6 +242,24 @@ int letkf_gpu_set_solver(letkf_gpu_t* h, float rtps, float rtpp, float mul);
  * and after the run's work with one group. */
 int letkf_gpu_run(letkf_gpu_t* h);
 
+/* letkf_gpu_run with the whole analysis captured as one HIP graph and
+ * replayed.  The first call (and the first after a buffer of the handle is
+ * reallocated or a setting changes: set_statevars, set_localizer,
+ * set_solver, set_tuning, more observations than the handle has room for)
+ * runs eagerly, which sizes the kept lists, then captures the launch
+ * sequence of the analysis on the handle's stream (search, list-space
+ * check, problem order, products, transform, post-processing: no host
+ * synchronisation inside); later calls replay it on the current inputs.
+ * New backgrounds (set_state*) and new observation sets that fit the
+ * handle's observation buffers (which keep a quarter of headroom over the
+ * largest set so far) are replayed without a new capture, which is how a
+ * cycled analysis runs.  If a replayed analysis needs more list space than
+ * the capture was sized for, it is redone eagerly with larger lists (and
+ * captured again on the next call).  The analysis is identical to
+ * letkf_gpu_run's; timings of a replay hold total_ms only.  No counterpart
+ * in the reference (it has no device work to capture). */
+int letkf_gpu_run_graph(letkf_gpu_t* h);
+
 /* Analysis in the reference's single-precision layout:
  * state_ij (k, ns, ij_count), mean_ij and sprd_ij (ns, ij_count).
  * Masked points hold the background perturbations, mean and spread. */

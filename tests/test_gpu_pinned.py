@@ -62,7 +62,7 @@ def _cases():
         synth.novrt(500e3), [dict(dep=2.0)], k=20, inc_max=np.array([0.1], np.float32))
     out["bounds"] = lambda: synth.point_problem(
         synth.novrt(500e3), [dict(dep=2.0)], k=20,
-        bounds=np.array([[-9.99e36], [0.05]], np.float32))
+        bounds=np.array([[-9.99e36], [15.05]], np.float32))    # background mean 15
     return out
 
 

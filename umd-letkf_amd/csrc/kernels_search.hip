@@ -383,7 +383,10 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     if (lane == 0) {
       s_sum[wid][0] = sr;
       s_hash[wid][0] = hs;
-      s_cnt[wid][0] = fd_cur;
+      // the kept count never exceeds the list space the count pass sized
+      // (the cover bound); a pair whose space was withdrawn (list capacity
+      // guard) ends with no list
+      s_cnt[wid][0] = fd_cur < fd_lim ? fd_cur : fd_lim;
     }
   }
   if (FILL && C.list_mode) {
@@ -391,7 +394,11 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     // whole 8-entry groups
     for (int g = 0; g < G; ++g) {
       int64_t end = 0;
-      if (lane == 0) end = s_cnt[wid][g];
+      if (lane == 0) {
+        end = s_cnt[wid][g];
+        const int64_t lim = PASS == kFillDiag ? fd_lim : s_lim[wid][g];
+        if (end > lim) end = lim;      // only past the space of a guarded pair
+      }
       end = __shfl(end, 0, 64);
       const int64_t pad_end = C.off[t * G + g] + PadCount()(end - C.off[t * G + g]);
       if (end + lane < pad_end) {
@@ -459,6 +466,39 @@ __global__ __launch_bounds__(256) void k_count_cover(LocDev L, ObsDev O, StateDe
       ncand += O.cell_start[b * kLonCells + rc1[r] + 1] - O.cell_start[b * kLonCells + rc0[r]];
   }
   C.cnt[t] = ncand;
+}
+
+// The graph-captured analysis.  Its counters are cleared by a kernel (a
+// captured graph holds no memset nodes), and the list space is checked on
+// the device instead of by the host: a (gridpoint, group) pair whose padded
+// list (plus the alignment slack of the vector loads) would end past the
+// list capacity gets no list space and no solve, and the overflow is
+// counted for the host, which reads the counters after the replay.
+__global__ void k_clear_u64(int64_t n, unsigned long long* p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_list_guard(int64_t np, int64_t* cnt, const int64_t* off, uint8_t* solve,
+                             int64_t cap, unsigned long long* ctr) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  if (off[p] + PadCount()(cnt[p]) + kListAlign > cap) {
+    cnt[p] = 0;
+    solve[p] = 0;
+    atomicAdd(ctr + kCtrOverflow, 1ull);
+  }
+}
+
+void launch_clear_u64(hipStream_t st, int64_t n, unsigned long long* p) {
+  if (n <= 0) return;
+  k_clear_u64<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, p);
+}
+
+void launch_list_guard(hipStream_t st, int64_t np, int64_t* cnt, const int64_t* off,
+                       uint8_t* solve, int64_t cap, unsigned long long* ctr) {
+  if (np <= 0) return;
+  k_list_guard<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(np, cnt, off, solve, cap, ctr);
 }
 
 void launch_search(hipStream_t st, bool fill, const LocDev& L, const ObsDev& O,

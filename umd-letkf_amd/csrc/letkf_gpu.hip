@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -125,6 +126,15 @@ struct letkf_gpu_t {
   int64_t* h_tot = nullptr;    // pinned
 
   letkf_gpu_timing timing{};
+  // captured analysis (letkf_gpu_run_graph): valid while gen == graph_gen.
+  // gen counts every reallocation of a handle buffer and every change of a
+  // setting the kernels take by value
+  uint64_t gen = 1, graph_gen = 0;
+  int graph_cfg = -1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int64_t obs_cap = 0;     // columns allocated for the observation buffers
+  int list_headroom_div = 4;   // lists grow to total + total / div
   // device memory of the handle: live allocations and an optional cap
   std::unordered_map<void*, size_t> live;
   size_t live_bytes = 0;
@@ -147,6 +157,7 @@ struct letkf_gpu_t {
 };
 
 static std::string g_create_err;
+
 
 static int fail(letkf_gpu_t* h, int code, const char* fmt, ...) {
   char buf[512];
@@ -190,11 +201,22 @@ static void raw_free(letkf_gpu_t* h, void* p) {
   }
   hipFree(p);
 }
+// a buffer pointer stored in the handle itself (not a local temporary)
+static bool handle_member(const letkf_gpu_t* h, const void* p) {
+  const char* a = (const char*)p;
+  return a >= (const char*)h && a < (const char*)(h + 1);
+}
 template <typename T>
 static hipError_t dalloc(letkf_gpu_t* h, T** p, size_t n) {
+  if (handle_member(h, p)) ++h->gen;     // a captured analysis no longer matches
   if (*p) { raw_free(h, *p); *p = nullptr; }
   if (n == 0) n = 1;
   return raw_alloc(h, (void**)p, n * sizeof(T));
+}
+// allocate a handle buffer of a size fixed for the handle's life once
+template <typename T>
+static hipError_t dalloc_once(letkf_gpu_t* h, T** p, size_t n) {
+  return *p ? hipSuccess : dalloc(h, p, n);
 }
 template <typename T>
 static void dfree(letkf_gpu_t* h, T*& p) {
@@ -274,6 +296,8 @@ int letkf_gpu_destroy(letkf_gpu_t* h) {
   if (!h) return LETKF_OK;
   hipStreamSynchronize(h->stream);
   hipStreamSynchronize(h->stream2);
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  if (h->graph) hipGraphDestroy(h->graph);
   dfree(h, h->d_state); dfree(h, h->d_mean); dfree(h, h->d_sprd); dfree(h, h->d_lat); dfree(h, h->d_lon);
   dfree(h, h->d_mask); dfree(h, h->d_gx); dfree(h, h->d_gy); dfree(h, h->d_gz);
   dfree(h, h->d_ox); dfree(h, h->d_oy); dfree(h, h->d_oz); dfree(h, h->d_ozdim); dfree(h, h->d_oerr);
@@ -302,6 +326,28 @@ int letkf_gpu_destroy(letkf_gpu_t* h) {
   return LETKF_OK;
 }
 
+}  // extern "C"
+
+// background buffers: sizes fixed by the handle's dims, allocated by the
+// first set_state* and reused by later ones (a new cycle's background keeps
+// the device addresses a captured analysis was recorded with)
+static hipError_t alloc_state(letkf_gpu_t* h) {
+  const size_t k = (size_t)h->k, ns = (size_t)h->ns, nij = (size_t)h->nij;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = dalloc_once(h, &h->d_state, k * ns * nij);
+  if (e == hipSuccess) e = dalloc_once(h, &h->d_mean, ns * nij);
+  if (e == hipSuccess) e = dalloc_once(h, &h->d_sprd, ns * nij);
+  if (e == hipSuccess) e = dalloc_once(h, &h->d_lat, nij);
+  if (e == hipSuccess) e = dalloc_once(h, &h->d_lon, nij);
+  if (e == hipSuccess) e = dalloc_once(h, &h->d_mask, nij);
+  if (e == hipSuccess) e = dalloc_once(h, &h->d_gx, nij);
+  if (e == hipSuccess) e = dalloc_once(h, &h->d_gy, nij);
+  if (e == hipSuccess) e = dalloc_once(h, &h->d_gz, nij);
+  return e;
+}
+
+extern "C" {
+
 int letkf_gpu_set_state_members(letkf_gpu_t* h, const float* members, const float* lat_ij,
                                 const float* lon_ij, const int32_t* mask_ij) {
   if (!h) return LETKF_EINVAL;
@@ -310,15 +356,7 @@ int letkf_gpu_set_state_members(letkf_gpu_t* h, const float* members, const floa
     return fail(h, LETKF_EINVAL, "null state array");
   if (k < 2) return fail(h, LETKF_EINVAL, "ens_size below 2: no spread");
   h->have_run = false;   // new inputs: the previous analysis is gone
-  HIPCHK(h, dalloc(h, &h->d_state, (size_t)(k * ns * nij)));
-  HIPCHK(h, dalloc(h, &h->d_mean, (size_t)(ns * nij)));
-  HIPCHK(h, dalloc(h, &h->d_sprd, (size_t)(ns * nij)));
-  HIPCHK(h, dalloc(h, &h->d_lat, (size_t)nij));
-  HIPCHK(h, dalloc(h, &h->d_lon, (size_t)nij));
-  HIPCHK(h, dalloc(h, &h->d_mask, (size_t)nij));
-  HIPCHK(h, dalloc(h, &h->d_gx, (size_t)nij));
-  HIPCHK(h, dalloc(h, &h->d_gy, (size_t)nij));
-  HIPCHK(h, dalloc(h, &h->d_gz, (size_t)nij));
+  HIPCHK(h, alloc_state(h));
   if (nij > 0) {
     float* raw = nullptr;
     HIPCHK(h, dalloc(h, &raw, (size_t)(k * ns * nij)));
@@ -351,15 +389,7 @@ int letkf_gpu_set_state(letkf_gpu_t* h, const float* state_ij, const float* mean
   if (nij > 0 && (!state_ij || !mean_ij || !sprd_ij || !lat_ij || !lon_ij || !mask_ij))
     return fail(h, LETKF_EINVAL, "null state array");
   h->have_run = false;
-  HIPCHK(h, dalloc(h, &h->d_state, (size_t)(k * ns * nij)));
-  HIPCHK(h, dalloc(h, &h->d_mean, (size_t)(ns * nij)));
-  HIPCHK(h, dalloc(h, &h->d_sprd, (size_t)(ns * nij)));
-  HIPCHK(h, dalloc(h, &h->d_lat, (size_t)nij));
-  HIPCHK(h, dalloc(h, &h->d_lon, (size_t)nij));
-  HIPCHK(h, dalloc(h, &h->d_mask, (size_t)nij));
-  HIPCHK(h, dalloc(h, &h->d_gx, (size_t)nij));
-  HIPCHK(h, dalloc(h, &h->d_gy, (size_t)nij));
-  HIPCHK(h, dalloc(h, &h->d_gz, (size_t)nij));
+  HIPCHK(h, alloc_state(h));
   if (nij > 0) {
     HIPCHK(h, hipMemcpyAsync(h->d_state, state_ij, sizeof(float) * k * ns * nij, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_mean, mean_ij, sizeof(float) * ns * nij, hipMemcpyHostToDevice, h->stream));
@@ -480,12 +510,24 @@ static int set_obs_impl(letkf_gpu_t* h, int64_t n, const float* lat, const float
     nn = ns1;
   }
   h->nobs = n;
-  OBSCHK(dalloc(h, &h->d_ox, nn)); OBSCHK(dalloc(h, &h->d_oy, nn)); OBSCHK(dalloc(h, &h->d_oz, nn));
-  OBSCHK(dalloc(h, &h->d_ozdim, nn)); OBSCHK(dalloc(h, &h->d_oerr, nn)); OBSCHK(dalloc(h, &h->d_odep, nn));
-  OBSCHK(dalloc(h, &h->d_otime, nn)); OBSCHK(dalloc(h, &h->d_oidx, nn)); OBSCHK(dalloc(h, &h->d_oobsid, nn));
-  OBSCHK(dalloc(h, &h->d_oplatid, nn)); OBSCHK(dalloc(h, &h->d_ohx, nn * k));
-  OBSCHK(dalloc(h, &h->d_cell_start, (size_t)kCells + 1));
-  OBSCHK(dalloc(h, &h->d_osurf, nn)); OBSCHK(dalloc(h, &h->d_otwp, nn)); OBSCHK(dalloc(h, &h->d_otws, nn));
+  h->kp = (int)(((k + 1) + 31) / 32 * 32);
+  if ((int64_t)nn > h->obs_cap) {
+    // grow-only, with a quarter of headroom (kept below the 2^31-byte range
+    // of the padded columns), so that later observation sets of similar size
+    // keep the device addresses a captured analysis was recorded with
+    const int64_t lim = ((1ll << 31) - 1) / (4ll * h->kp);
+    const int64_t cap = std::max<int64_t>((int64_t)nn,
+                                          std::min<int64_t>((int64_t)nn + (int64_t)nn / 4, lim));
+    const size_t c = (size_t)cap;
+    OBSCHK(dalloc(h, &h->d_ox, c)); OBSCHK(dalloc(h, &h->d_oy, c)); OBSCHK(dalloc(h, &h->d_oz, c));
+    OBSCHK(dalloc(h, &h->d_ozdim, c)); OBSCHK(dalloc(h, &h->d_oerr, c)); OBSCHK(dalloc(h, &h->d_odep, c));
+    OBSCHK(dalloc(h, &h->d_otime, c)); OBSCHK(dalloc(h, &h->d_oidx, c)); OBSCHK(dalloc(h, &h->d_oobsid, c));
+    OBSCHK(dalloc(h, &h->d_oplatid, c)); OBSCHK(dalloc(h, &h->d_ohx, c * k));
+    OBSCHK(dalloc(h, &h->d_osurf, c)); OBSCHK(dalloc(h, &h->d_otwp, c)); OBSCHK(dalloc(h, &h->d_otws, c));
+    OBSCHK(dalloc(h, &h->d_ohxp, c * h->kp));
+    h->obs_cap = cap;
+  }
+  OBSCHK(dalloc_once(h, &h->d_cell_start, (size_t)kCells + 1));
   OBSCHK(dalloc(h, &cell, nn)); OBSCHK(dalloc(h, &scell, nn));
   OBSCHK(dalloc(h, &iota, nn)); OBSCHK(dalloc(h, &perm, nn));
   if (n > 0) {
@@ -501,8 +543,6 @@ static int set_obs_impl(letkf_gpu_t* h, int64_t n, const float* lat, const float
                     h->d_ozdim, h->d_oerr, h->d_odep, h->d_oidx, h->d_oobsid,
                     h->d_oplatid, h->d_otime, h->d_ohx, h->d_cell_start);
   OBSCHK(hipGetLastError());
-  h->kp = (int)(((k + 1) + 31) / 32 * 32);
-  OBSCHK(dalloc(h, &h->d_ohxp, nn * h->kp));
   launch_obs_pad_hx(st, n, (int)k, h->kp, h->d_ohx, h->d_odep, h->d_ohxp);
   OBSCHK(hipGetLastError());
   OBSCHK(hipStreamSynchronize(st));
@@ -575,6 +615,7 @@ int letkf_gpu_set_statevars(letkf_gpu_t* h, int32_t nvars, const int32_t* s_idx,
                   s_idx[v], s_idx[v] + levels[v], (long long)h->ns);
   }
   h->have_run = false;
+  ++h->gen;
   h->nvars = nvars;
   h->var_s.assign(s_idx, s_idx + nvars);
   h->var_lev.assign(levels, levels + nvars);
@@ -612,6 +653,7 @@ static int upload_flags(letkf_gpu_t* h) {
 int letkf_gpu_set_localizer(letkf_gpu_t* h, const letkf_loc_desc* d) {
   if (!h || !d) return LETKF_EINVAL;
   h->have_run = false;
+  ++h->gen;
   LocDev L{};
   L.cls = d->cls;
   switch (d->cls) {
@@ -688,6 +730,7 @@ int letkf_gpu_set_solver(letkf_gpu_t* h, float rtps, float rtpp, float mul) {
     return fail(h, LETKF_EINVAL, "cannot have both RTPS and RTPP enabled at the same time. ");
   if (mul < 1.0f) return fail(h, LETKF_EINVAL, "infl_mul must be >=1.0.");
   h->have_run = false;
+  ++h->gen;
   h->rtps = rtps;
   h->rtpp = rtpp;
   h->mul = mul;
@@ -843,11 +886,11 @@ static int ensure_outputs(letkf_gpu_t* h) {
   return LETKF_OK;
 }
 
-int letkf_gpu_run(letkf_gpu_t* h) {
-  if (!h) return LETKF_EINVAL;
-  // whatever happens below, the previous analysis no longer matches the
-  // inputs: a run that fails leaves no analysis to fetch
-  h->have_run = false;
+}  // extern "C"
+
+// The checks of every run, before the analysis buffers are touched, and the
+// buffers themselves.
+static int run_prepare(letkf_gpu_t* h) {
   if (!h->have_state || !h->have_obs || !h->have_loc || !h->have_solver || !h->have_vars)
     return fail(h, LETKF_ESTATE, "set_state, set_obs, set_statevars, set_localizer and "
                                  "set_solver must precede run");
@@ -864,22 +907,94 @@ int letkf_gpu_run(letkf_gpu_t* h) {
   if (!mat && eigen_lds_bytes((int)h->k) + 8192 + 64 > 150 * 1024)
     return fail(h, LETKF_EUNSUPPORTED, "ens_size %lld too large for the FP64 eigensolver",
                 (long long)h->k);
-  if (h->batched() && h->nobs * (int64_t)h->kp * 4 >= (1ll << 31))
+  if (h->batched() && h->obs_cap * (int64_t)h->kp * 4 >= (1ll << 31))
     return fail(h, LETKF_EUNSUPPORTED, "batched path: n_obs x %d rows above 2^29 values",
                 h->kp);
-  int rc = ensure_outputs(h);
-  if (rc) return rc;
+  return ensure_outputs(h);
+}
+
+// the device views of the handle's buffers that every kernel takes
+struct RunViews {
+  StateDev S;
+  ObsDev O;
+  VarsDev V;
+  DiagDev D;
+};
+static RunViews run_views(const letkf_gpu_t* h) {
+  // the observation count the kernels see is the allocated capacity (the
+  // padded columns' buffer range); lists only reference the first nobs
+  return RunViews{
+      StateDev{h->k, h->ns, h->nij, h->d_state, h->d_mean, h->d_sprd, h->d_lat, h->d_lon,
+               h->d_mask, h->d_gx, h->d_gy, h->d_gz},
+      ObsDev{h->obs_cap, h->d_ox, h->d_oy, h->d_oz, h->d_ozdim, h->d_oerr, h->d_odep,
+             h->d_oidx, h->d_osurf, h->d_otwp, h->d_otws, h->d_ohx, h->d_cell_start,
+             h->d_ohxp, h->kp},
+      VarsDev{h->nvars, h->d_slab_var, h->d_slab_lev, h->d_inc_max, h->d_bounds,
+              h->d_lev_depth},
+      DiagDev{h->d_maxhz, h->d_obscount, h->d_lgm, h->d_lgloc, h->d_lghash, h->d_mld,
+              h->d_ctr}};
+}
+
+// the transform of one chunk's problems and its post-processing, on st
+static hipError_t enqueue_solve(letkf_gpu_t* h, hipStream_t st, const RunViews& W,
+                                const ChunkDev& C) {
+  if (h->use_mat) {
+    const RatTab R{h->d_rat, h->d_ratn, kRatBuckets};
+    launch_transform_mat(st, h->batched() ? h->dims.precision : LETKF_PREC_FP64, h->loc, W.S,
+                         W.V, C, W.D, h->d_B, h->d_b, h->d_tau, h->d_dd, h->d_ee, R, h->d_ana);
+  } else if (h->eigK) {
+    RatTab R;
+    if (h->use_rat) { R.tab = h->d_rat; R.n = h->d_ratn; R.nb = kRatBuckets; }
+    launch_eigen_pipeline(st, h->loc, W.S, W.V, C, W.D, h->d_B, h->d_b, h->d_tau, h->d_dd,
+                          h->d_ee, h->d_rot, h->cap_rot, h->d_itr, h->cap_it, h->d_nit,
+                          h->d_status, h->d_Z, h->vec_nv, h->d_U, h->d_nvec, R, h->d_ana,
+                          h->tune_trireg != 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // problems the QL could not finish go to the Jacobi solver
+    e = launch_eigen_apply_f64(st, h->loc, W.S, W.V, C, W.D, h->d_B, h->d_b, h->d_status,
+                               h->d_ana);
+    if (e != hipSuccess) return e;
+  } else {
+    const hipError_t e = launch_eigen_apply_f64(st, h->loc, W.S, W.V, C, W.D, h->d_B, h->d_b,
+                                                nullptr, h->d_ana);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+// The search counters of the last analysis (work counts into T), the
+// max_obs capacity check and, for a replayed graph, the pairs refused list
+// space (*overflow; nullptr: not looked at).
+static int read_counters(letkf_gpu_t* h, letkf_gpu_timing* T, int64_t* overflow) {
+  const size_t ctr_bytes = sizeof(unsigned long long) * kCtrSlots * kCtrStride;
+  HIPCHK(h, hipMemcpyAsync(h->h_ctr, h->d_ctr, ctr_bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  unsigned long long maxc = 0;
+  for (int sl = 0; sl < kCtrSlots; ++sl) {
+    T->n_problems += (int64_t)h->h_ctr[sl * kCtrStride + kCtrSolves];
+    T->n_cand += (int64_t)h->h_ctr[sl * kCtrStride + kCtrCand];
+    T->n_pairs += (int64_t)h->h_ctr[sl * kCtrStride + kCtrPairs];
+    maxc = std::max(maxc, h->h_ctr[sl * kCtrStride + kCtrMax]);
+  }
+  if (overflow) *overflow = (int64_t)h->h_ctr[kCtrOverflow];
+  if ((long long)maxc > (long long)h->dims.max_obs) {
+    return fail(h, LETKF_ECAPACITY, "a gridpoint has %llu observations in its search "
+                "radius, above max_obs = %lld", maxc, (long long)h->dims.max_obs);
+  }
+  return LETKF_OK;
+}
+
+static int run_eager(letkf_gpu_t* h) {
+  int rc = LETKF_OK;
   hipStream_t st = h->stream;
   const int64_t k = h->k, G = h->G;
 
-  StateDev S{h->k, h->ns, h->nij, h->d_state, h->d_mean, h->d_sprd, h->d_lat, h->d_lon,
-             h->d_mask, h->d_gx, h->d_gy, h->d_gz};
-  ObsDev O{h->nobs, h->d_ox, h->d_oy, h->d_oz, h->d_ozdim, h->d_oerr, h->d_odep,
-           h->d_oidx, h->d_osurf, h->d_otwp, h->d_otws, h->d_ohx, h->d_cell_start,
-           h->d_ohxp, h->kp};
-  VarsDev V{h->nvars, h->d_slab_var, h->d_slab_lev, h->d_inc_max, h->d_bounds, h->d_lev_depth};
-  DiagDev D{h->d_maxhz, h->d_obscount, h->d_lgm, h->d_lgloc, h->d_lghash, h->d_mld,
-            h->d_ctr};
+  const RunViews W = run_views(h);
+  const StateDev& S = W.S;
+  const ObsDev& O = W.O;
+  const VarsDev& V = W.V;
+  const DiagDev& D = W.D;
   const size_t ctr_bytes = sizeof(unsigned long long) * kCtrSlots * kCtrStride;
   HIPCHK(h, hipMemsetAsync(h->d_ctr, 0, ctr_bytes, st));
 
@@ -965,8 +1080,9 @@ int letkf_gpu_run(letkf_gpu_t* h) {
       }
     }
     if (total > b_list_cap) {
-      // this set's previous chunk is finished (the search stream waited for it)
-      const int64_t cap = total + total / 4 + 1024;
+      // this set's previous chunk is finished (the search stream waited for it);
+      // a quarter of headroom, half for a run that sizes a graph capture
+      const int64_t cap = total + total / (h->list_headroom_div) + 1024;
       RUNCHK(dalloc(h, &b_list_idx, (size_t)cap));
       RUNCHK(dalloc(h, &b_list_w, (size_t)cap));
       b_list_cap = cap;
@@ -992,25 +1108,7 @@ int letkf_gpu_run(letkf_gpu_t* h) {
     }
     RUNCHK(hipGetLastError());
     RUNCHK(hipEventRecord(ce[4], st));
-    if (h->use_mat) {
-      const RatTab R{h->d_rat, h->d_ratn, kRatBuckets};
-      launch_transform_mat(st, h->batched() ? h->dims.precision : LETKF_PREC_FP64, h->loc, S,
-                           V, C, D, h->d_B, h->d_b, h->d_tau, h->d_dd, h->d_ee, R, h->d_ana);
-    } else if (h->eigK) {
-      RatTab R;
-      if (h->use_rat) { R.tab = h->d_rat; R.n = h->d_ratn; R.nb = kRatBuckets; }
-      launch_eigen_pipeline(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_tau, h->d_dd,
-                            h->d_ee, h->d_rot, h->cap_rot, h->d_itr, h->cap_it, h->d_nit,
-                            h->d_status, h->d_Z, h->vec_nv, h->d_U, h->d_nvec, R, h->d_ana,
-                            h->tune_trireg != 0);
-      RUNCHK(hipGetLastError());
-      // problems the QL could not finish go to the Jacobi solver
-      RUNCHK(launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, h->d_status,
-                                    h->d_ana));
-    } else {
-      RUNCHK(launch_eigen_apply_f64(st, h->loc, S, V, C, D, h->d_B, h->d_b, nullptr, h->d_ana));
-    }
-    RUNCHK(hipGetLastError());
+    RUNCHK(enqueue_solve(h, st, W, C));
     RUNCHK(hipEventRecord(ce[5], st));
     launch_finalize(st, h->loc, S, V, C, D, h->rtps, h->rtpp, h->mul, h->d_ana, h->d_ana,
                     h->d_amean, h->d_asprd);
@@ -1043,24 +1141,167 @@ int letkf_gpu_run(letkf_gpu_t* h) {
   }
   cleanup_events();
 #undef RUNCHK
-  {
-    HIPCHK(h, hipMemcpyAsync(h->h_ctr, h->d_ctr, ctr_bytes, hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipStreamSynchronize(st));
-    unsigned long long maxc = 0;
-    for (int sl = 0; sl < kCtrSlots; ++sl) {
-      T.n_problems += (int64_t)h->h_ctr[sl * kCtrStride + kCtrSolves];
-      T.n_cand += (int64_t)h->h_ctr[sl * kCtrStride + kCtrCand];
-      T.n_pairs += (int64_t)h->h_ctr[sl * kCtrStride + kCtrPairs];
-      maxc = std::max(maxc, h->h_ctr[sl * kCtrStride + kCtrMax]);
-    }
-    if ((long long)maxc > (long long)h->dims.max_obs) {
-      return fail(h, LETKF_ECAPACITY, "a gridpoint has %llu observations in its search "
-                  "radius, above max_obs = %lld", maxc, (long long)h->dims.max_obs);
-    }
-  }
+  rc = read_counters(h, &T, nullptr);
+  if (rc) return rc;
   h->timing = T;
   h->have_run = true;
   return LETKF_OK;
+}
+
+// The whole analysis as one launch sequence on h->stream with no host
+// interaction: counters cleared by a kernel, each chunk's list space
+// checked on the device (k_list_guard) against the capacity an earlier
+// eager run sized, the search of each chunk, its problem order, products,
+// transform and post-processing.  Captured into a graph by run_graph.
+static hipError_t enqueue_analysis(letkf_gpu_t* h) {
+  hipStream_t st = h->stream;
+  const RunViews W = run_views(h);
+  const int64_t G = h->G;
+  launch_clear_u64(st, (int64_t)kCtrSlots * kCtrStride, h->d_ctr);
+  hipError_t e = hipGetLastError();
+  for (int64_t i0 = 0; i0 < h->nij && e == hipSuccess; i0 += h->chunk_gp) {
+    const int64_t ngp = std::min(h->chunk_gp, h->nij - i0);
+    const int64_t np = ngp * G;
+    ChunkDev C{i0, ngp, (int)G, h->d_cnt, h->d_off, h->d_solve, h->d_list_idx, h->d_list_w,
+               h->batched() ? 1 : 0, h->tune_xcd};
+    launch_search(st, false, h->loc, W.O, W.S, W.V, C, W.D);
+    e = hipGetLastError();
+    if (e != hipSuccess) break;
+    size_t tb = h->tmp_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum(h->d_tmp, tb, padded_counts(h->d_cnt), h->d_off,
+                                         (int)np, st);
+    if (e != hipSuccess) break;
+    launch_list_guard(st, np, h->d_cnt, h->d_off, h->d_solve, h->list_cap, h->d_ctr);
+    launch_search(st, true, h->loc, W.O, W.S, W.V, C, W.D);
+    e = hipGetLastError();
+    if (e != hipSuccess) break;
+    if (h->batched()) {
+      launch_bucket_keys(st, h->tune_bucket, h->tune_bbits, np, (int)G, i0, h->d_cnt,
+                         h->d_solve, h->d_lat, h->d_lon, h->d_bkey, h->d_bval);
+      size_t sb = h->stmp_bytes;
+      e = hipcub::DeviceRadixSort::SortPairs(h->d_stmp, sb, h->d_bkey, h->d_bkey2, h->d_bval,
+                                             h->d_perm, (int)np, 0, 32, st);
+      if (e != hipSuccess) break;
+      launch_product_mx(st, h->dims.precision, W.O, W.S, C, h->d_perm, h->d_B, h->d_b);
+    } else {
+      launch_product_f64(st, W.O, W.S, C, h->d_B, h->d_b);
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess) e = enqueue_solve(h, st, W, C);
+    if (e != hipSuccess) break;
+    launch_finalize(st, h->loc, W.S, W.V, C, W.D, h->rtps, h->rtpp, h->mul, h->d_ana, h->d_ana,
+                    h->d_amean, h->d_asprd);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+// an eager run on the main stream alone: the graph uses the first set of
+// chunk buffers only, so their lists are sized by such a run
+static int run_eager_single(letkf_gpu_t* h) {
+  const int ov = h->tune_overlap;
+  h->tune_overlap = 0;
+  h->list_headroom_div = 2;
+  const int rc = run_eager(h);
+  h->tune_overlap = ov;
+  h->list_headroom_div = 4;
+  return rc;
+}
+
+// the launch choices a captured graph was recorded under (besides gen)
+static int graph_config(const letkf_gpu_t* h) {
+  const char* ev = getenv("LETKF_GPU_EIGEN");
+  const int jac = ev && strcmp(ev, "jacobi") == 0;
+  return jac | (h->eigK << 1) | ((int)h->use_mat << 9) | ((int)h->use_rat << 10) |
+         (h->vec_nv << 11);
+}
+
+static void drop_graph(letkf_gpu_t* h) {
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  h->graph_gen = 0;
+}
+
+static int run_graph(letkf_gpu_t* h) {
+  hipStream_t st = h->stream;
+  if (h->gexec && h->graph_gen == h->gen && h->graph_cfg == graph_config(h)) {
+    // replay: the whole analysis in one launch, timed as a whole
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(h, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+      hipEventDestroy(e0);
+      return fail(h, LETKF_EHIP, "hipEventCreate failed");
+    }
+    hipError_t e = hipEventRecord(e0, st);
+    if (e == hipSuccess) e = hipGraphLaunch(h->gexec, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    letkf_gpu_timing T{};
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    HIPCHK(h, e);
+    T.total_ms = ms;
+    T.n_launch = 1;
+    int64_t overflow = 0;
+    const int rc = read_counters(h, &T, &overflow);
+    if (rc) return rc;
+    if (overflow > 0) {
+      // this analysis needed more list space than the capture had: run it
+      // eagerly (which grows the lists, so the next call captures again)
+      return run_eager_single(h);
+    }
+    h->timing = T;
+    h->have_run = true;
+    return LETKF_OK;
+  }
+  // first call, or buffers / settings changed: an eager run sizes and
+  // allocates everything (and gives this call's analysis), then the launch
+  // sequence is captured for the calls that follow
+  drop_graph(h);
+  int rc = run_eager_single(h);
+  if (rc) return rc;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    const hipError_t ee = enqueue_analysis(h);
+    e = hipStreamEndCapture(st, &g);
+    if (e == hipSuccess) e = ee;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    h->gexec = nullptr;
+    h->have_run = false;
+    return fail(h, LETKF_EHIP, "capturing the analysis as a graph: %s", hipGetErrorString(e));
+  }
+  h->graph = g;
+  h->graph_gen = h->gen;
+  h->graph_cfg = graph_config(h);
+  return LETKF_OK;
+}
+
+extern "C" {
+
+int letkf_gpu_run(letkf_gpu_t* h) {
+  if (!h) return LETKF_EINVAL;
+  // whatever happens below, the previous analysis no longer matches the
+  // inputs: a run that fails leaves no analysis to fetch
+  h->have_run = false;
+  const int rc = run_prepare(h);
+  if (rc) return rc;
+  return run_eager(h);
+}
+
+int letkf_gpu_run_graph(letkf_gpu_t* h) {
+  if (!h) return LETKF_EINVAL;
+  h->have_run = false;
+  const int rc = run_prepare(h);
+  if (rc) return rc;
+  return run_graph(h);
 }
 
 int letkf_gpu_get_state_f64(letkf_gpu_t* h, double* state, double* mean, double* sprd) {
@@ -1250,6 +1491,7 @@ int letkf_gpu_set_tuning(letkf_gpu_t* h, const letkf_gpu_tuning* t) {
     return fail(h, LETKF_EINVAL, "tuning overlap %d not 0 or 1", t->overlap);
   if (t->chunk_log2 < 17 || t->chunk_log2 > 20)
     return fail(h, LETKF_EINVAL, "tuning chunk_log2 %d outside 17..20", t->chunk_log2);
+  ++h->gen;
   h->tune_bucket = t->order;
   h->tune_bbits = t->bucket_bits;
   h->tune_trireg = t->tridiag_reg;

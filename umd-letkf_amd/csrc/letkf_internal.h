@@ -122,6 +122,7 @@ struct DiagDev {
 constexpr int kCtrSlots = 64;
 constexpr int kCtrStride = 64;        // 512 B per slot
 constexpr int kCtrSolves = 0, kCtrCand = 16, kCtrPairs = 32, kCtrMax = 48;
+constexpr int kCtrOverflow = 8;       // slot 0 only: pairs refused list space
 
 // group of slab s at a gridpoint (letkf_loc_ocean.F90:360-375 for bkg_t)
 __device__ __forceinline__ int slab_group(const LocDev& L, const VarsDev& V,
@@ -173,6 +174,9 @@ void launch_obs_compact(hipStream_t st, int64_t nsel, int k, const int32_t* sel,
                         const ObsFields& in, const ObsFields& out);
 void launch_grid_xyz(hipStream_t st, int64_t n, const float* lat,
                      const float* lon, double* x, double* y, double* z);
+void launch_clear_u64(hipStream_t st, int64_t n, unsigned long long* p);
+void launch_list_guard(hipStream_t st, int64_t np, int64_t* cnt, const int64_t* off,
+                       uint8_t* solve, int64_t cap, unsigned long long* ctr);
 void launch_search(hipStream_t st, bool fill, const LocDev& L,
                    const ObsDev& O, const StateDev& S, const VarsDev& V,
                    const ChunkDev& C, const DiagDev& D);

@@ -118,6 +118,11 @@ class LetkfGpuSolver:
     def run(self):
         self._chk(self.lib.letkf_gpu_run(self.h))
 
+    def run_graph(self):
+        """The analysis captured once as a HIP graph and replayed by later
+        calls (letkf_gpu_run_graph); same results as run()."""
+        self._chk(self.lib.letkf_gpu_run_graph(self.h))
+
     def maxgroups(self) -> int:
         g = C.c_int32()
         self._chk(self.lib.letkf_gpu_maxgroups(self.h, C.byref(g)))

@@ -635,11 +635,12 @@ def point_problem(loc, obs, var_lev=(1,), mean_profile=None, k=10, seed=0x1E7F03
     ``obs`` is a list of dicts with keys dist (m), bearing (deg), obsid,
     platid, zdim, time, dep (O-F), err and slab (the slab whose members are
     the obs' hx).  ``mean_profile`` (ns,) fixes the background mean of the
-    column (the mixed-layer level reads it); every slab's perturbations are
-    one common draw, so each observation correlates with every slab."""
+    column (the mixed-layer level reads it; default 15 everywhere, an SST-like
+    value); every slab's perturbations are one common draw, so each
+    observation correlates with every slab."""
     g = rng(seed)
     ns = int(sum(var_lev))
-    base = np.zeros(ns) if mean_profile is None else np.asarray(mean_profile, float)
+    base = np.full(ns, 15.0) if mean_profile is None else np.asarray(mean_profile, float)
     e = g.standard_normal(k)
     e = 0.5 * (e - e.mean()) / e.std()
     members = (base[None, :, None] + e[:, None, None]).astype(np.float32)
