@@ -273,6 +273,20 @@ int letkf_gpu_get_state(letkf_gpu_t* h, float* state_ij, float* mean_ij,
 int letkf_gpu_get_state_members(letkf_gpu_t* h, float* members, float* mean_ij,
                                 float* sprd_ij);
 
+/* letkf_gpu_get_state_members without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (tight serving or cycling loops, harnesses that forbid
+ * hipMalloc).  The file-order conversion is staged through a 64 MiB device
+ * array owned by the library (one per GPU, held until the process ends) in
+ * slices of whole gridpoints, each copied down before the next; any
+ * ij_count works, but one gridpoint's column (4 (k ns + 2 ns) bytes) must
+ * fit in the array (LETKF_ECAPACITY otherwise).  Results are identical to
+ * letkf_gpu_get_state_members.  Every handle on a GPU shares that GPU's
+ * array: calls that overlap in time on one GPU (from several host threads)
+ * race.  Any pointer may be NULL. */
+int letkf_gpu_get_state_members_noalloc(letkf_gpu_t* h, float* members, float* mean_ij,
+                                        float* sprd_ij);
+
 /* The same analysis at the precision it was computed in (FP64 for
  * LETKF_PREC_FP64; the FP32 results widened otherwise). */
 int letkf_gpu_get_state_f64(letkf_gpu_t* h, double* state_ij,

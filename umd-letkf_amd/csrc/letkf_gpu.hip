@@ -126,6 +126,9 @@ struct letkf_gpu_t {
   int64_t* h_tot = nullptr;    // pinned
 
   letkf_gpu_timing timing{};
+  // device address of this device's no-allocation staging array
+  void* d_scratch = nullptr;
+  hipError_t scratch_err = hipSuccess;
   // captured analysis (letkf_gpu_run_graph): valid while gen == graph_gen.
   // gen counts every reallocation of a handle buffer and every change of a
   // setting the kernels take by value
@@ -158,6 +161,40 @@ struct letkf_gpu_t {
 
 static std::string g_create_err;
 
+// ---------------------------------------------------------------------------
+// Staging memory of letkf_gpu_get_state_members_noalloc: a file-scope device
+// array, so that the call allocates nothing.  Every GPU the library runs on
+// holds its own copy (64 MiB) until the process ends.  In host code the
+// array's name is not a device address; its address on each device is
+// looked up once per device, with that device current, by
+// hipGetSymbolAddress at the first letkf_gpu_create there, and only that
+// address is handed to kernels and copies.  All calls on one device share
+// the array: calls that overlap in time on one GPU race.
+// ---------------------------------------------------------------------------
+static constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// device address of `scratch` on the current device `dev` (looked up once)
+void* scratch_address(int dev, hipError_t* err) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::call_once(g_scratch_once[dev], [dev] {
+    void* p = nullptr;
+    g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? p : nullptr;
+  });
+  *err = g_scratch_err[dev];
+  return g_scratch_ptr[dev];
+}
+}  // namespace
 
 static int fail(letkf_gpu_t* h, int code, const char* fmt, ...) {
   char buf[512];
@@ -267,6 +304,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   if (const char* e = getenv("LETKF_GPU_CHUNK_LOG2"))
     h->tune_chunk_log2 = std::min(20, std::max(17, atoi(e)));
   hipGetDevice(&h->device);
+  h->d_scratch = scratch_address(h->device, &h->scratch_err);
   // both streams at the default priority
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
@@ -1341,6 +1379,58 @@ int letkf_gpu_get_state_members(letkf_gpu_t* h, float* members, float* mean, flo
     e = hipMemcpyAsync(sprd, dsp, 4 * ns * nij, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   HIPCHK(h, done(e));
+  return LETKF_OK;
+}
+
+int letkf_gpu_get_state_members_noalloc(letkf_gpu_t* h, float* members, float* mean,
+                                        float* sprd) {
+  if (!h) return LETKF_EINVAL;
+  if (!h->have_run) return fail(h, LETKF_ESTATE, "no analysis: call letkf_gpu_run first");
+  if (!h->d_scratch)
+    return fail(h, LETKF_EHIP, "no-allocation staging array: hipGetSymbolAddress failed (%s)",
+                hipGetErrorString(h->scratch_err));
+  const int64_t k = h->k, ns = h->ns, nij = h->nij;
+  if (nij * ns <= 0) return LETKF_OK;
+  // one gridpoint's column (members, mean, spread) must fit the array
+  const int64_t col_bytes = 4 * (k * ns + 2 * ns);
+  if (col_bytes > (int64_t)kNoAllocScratchBytes)
+    return fail(h, LETKF_ECAPACITY, "a gridpoint column of %lld bytes exceeds the %zu-byte "
+                "no-allocation staging array", (long long)col_bytes, kNoAllocScratchBytes);
+  // gridpoints per slice: as many columns as fit, optionally fewer
+  // (LETKF_GPU_NOALLOC_SLICE, gridpoints; used to test ragged slices)
+  int64_t slice = (int64_t)kNoAllocScratchBytes / col_bytes;
+  if (const char* e = getenv("LETKF_GPU_NOALLOC_SLICE")) {
+    const long long v = atoll(e);
+    if (v > 0) slice = std::min<int64_t>(slice, v);
+  }
+  float* sm = (float*)h->d_scratch;   // (c, ns, k) file-order members of the slice
+  hipError_t e = hipSuccess;
+  for (int64_t i0 = 0; i0 < nij && e == hipSuccess; i0 += slice) {
+    const int64_t c = std::min(slice, nij - i0);
+    float* smu = sm + c * ns * k;      // (c, ns) mean
+    float* ssp = smu + c * ns;         // (c, ns) spread
+    // every value of the three staging regions is written before it is
+    // copied, so nothing left by an earlier call needs clearing
+    launch_members_from_state(h->stream, c, ns, (int)k, h->d_ana + i0 * k * ns,
+                              h->d_amean + i0 * ns, h->d_asprd + i0 * ns,
+                              members ? sm : nullptr, mean ? smu : nullptr,
+                              sprd ? ssp : nullptr);
+    e = hipGetLastError();
+    // rows of c gridpoints go to column i0 of the caller's (nij, rows) arrays
+    if (e == hipSuccess && members)
+      e = hipMemcpy2DAsync(members + i0, 4 * nij, sm, 4 * c, 4 * c, (size_t)(ns * k),
+                           hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && mean)
+      e = hipMemcpy2DAsync(mean + i0, 4 * nij, smu, 4 * c, 4 * c, (size_t)ns,
+                           hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && sprd)
+      e = hipMemcpy2DAsync(sprd + i0, 4 * nij, ssp, 4 * c, 4 * c, (size_t)ns,
+                           hipMemcpyDeviceToHost, h->stream);
+    // the next slice overwrites the staging array
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  }
+  if (e != hipSuccess)
+    return fail(h, LETKF_EHIP, "get_state_members_noalloc: %s", hipGetErrorString(e));
   return LETKF_OK;
 }
 

@@ -98,7 +98,7 @@ EXPORTED = [
     "letkf_gpu_get_selection", "letkf_gpu_get_timing",
     "letkf_gpu_set_state_members", "letkf_gpu_set_obs_members",
     "letkf_gpu_get_state_members", "letkf_gpu_autotune", "letkf_gpu_get_tuning",
-    "letkf_gpu_set_tuning", "letkf_gpu_run_graph",
+    "letkf_gpu_set_tuning", "letkf_gpu_run_graph", "letkf_gpu_get_state_members_noalloc",
 ]
 
 _FP = C.POINTER(C.c_float)
@@ -129,6 +129,7 @@ _SIGS = {
     "letkf_gpu_set_obs_members": (C.c_int, [_H, C.c_int64, _FP, _FP, _FP, _FP, _FP,
                                             _FP, _IP, _IP, _IP, _FP]),
     "letkf_gpu_get_state_members": (C.c_int, [_H, _FP, _FP, _FP]),
+    "letkf_gpu_get_state_members_noalloc": (C.c_int, [_H, _FP, _FP, _FP]),
     "letkf_gpu_run_graph": (C.c_int, [_H]),
     "letkf_gpu_autotune": (C.c_int, [_H, C.c_int32, C.POINTER(Tuning)]),
     "letkf_gpu_get_tuning": (C.c_int, [_H, C.POINTER(Tuning)]),

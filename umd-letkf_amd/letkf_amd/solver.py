@@ -152,6 +152,18 @@ class LetkfGpuSolver:
                                                        abi.ptr(mu, F), abi.ptr(sp, F)))
         return mem, mu, sp
 
+    def state_members_noalloc(self):
+        """state_members through letkf_gpu_get_state_members_noalloc (no
+        device allocation; the library's staging array)."""
+        p = self.prob
+        mem = np.zeros((p.nij, p.ns, p.k), np.float32, order="F")
+        mu = np.zeros((p.nij, p.ns), np.float32, order="F")
+        sp = np.zeros((p.nij, p.ns), np.float32, order="F")
+        F = C.c_float
+        self._chk(self.lib.letkf_gpu_get_state_members_noalloc(
+            self.h, abi.ptr(mem, F), abi.ptr(mu, F), abi.ptr(sp, F)))
+        return mem, mu, sp
+
     def state_f32(self):
         p = self.prob
         st = np.zeros((p.k, p.ns, p.nij), np.float32, order="F")
