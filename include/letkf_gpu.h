@@ -48,6 +48,10 @@ extern "C" {
 #define LETKF_PREC_BF16X3   2  /* products split-BF16 (3 MFMA terms), FP32
                                   accumulation; eigensolve in FP64          */
 #define LETKF_PREC_BF16     3  /* products plain BF16, FP32 accumulation     */
+#define LETKF_PREC_BF16X4   4  /* products split-BF16 with all 4 MFMA terms,
+                                  compensated FP32 accumulation; eigensolve
+                                  in FP64 (FP32 products are compensated
+                                  the same way)                             */
 
 /* ---- execution path ---------------------------------------------------- */
 #define LETKF_PATH_PLAIN    0  /* one gridpoint per workgroup, unbucketed   */

@@ -2,8 +2,9 @@
 
 Selection is the same search as the FP64 path, so it must stay bit-exact.
 Analysis errors (max |gpu - oracle| / max |oracle|) are bounded per
-precision of the products: FP32 and split-BF16 (bf16x3) within 1e-5 on the
-mean and spread, plain BF16 within 1e-2 (reported, not a target).
+precision of the products: FP32, split-BF16 (bf16x3) and split-BF16 with
+all four terms (bf16x4) within 1e-5 on the mean and spread, plain BF16
+within 1e-2 (reported, not a target).
 """
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ from letkf_amd import synth
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp32": 1e-5, "bf16x3": 1e-5, "bf16": 1e-2}
+TOL = {"fp32": 1e-5, "bf16x3": 1e-5, "bf16x4": 1e-5, "bf16": 1e-2}
 
 
 def _mixed(prob, prec, points=None):
@@ -30,12 +31,12 @@ def _mixed(prob, prec, points=None):
     return errs
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16x4", "bf16"])
 def test_config1(prec):
     _mixed(synth.config1(), prec)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16x4", "bf16"])
 @pytest.mark.parametrize("k", [20, 49, 50])
 def test_random(prec, k):
     """k = 49 and 50 take the 50-row tridiagonalization and rational solve

@@ -277,7 +277,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
                 (long long)d->ens_size, kMaxK - 1);
   if (d->grid_ns < 1 || d->ij_count < 0)
     return fail(nullptr, LETKF_EINVAL, "invalid grid_ns / ij_count");
-  if (d->precision < LETKF_PREC_FP64 || d->precision > LETKF_PREC_BF16)
+  if (d->precision < LETKF_PREC_FP64 || d->precision > LETKF_PREC_BF16X4)
     return fail(nullptr, LETKF_EINVAL, "invalid precision %d", d->precision);
   if (d->path != LETKF_PATH_PLAIN && d->path != LETKF_PATH_BATCHED)
     return fail(nullptr, LETKF_EINVAL, "invalid path %d", d->path);

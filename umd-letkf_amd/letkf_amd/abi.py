@@ -26,8 +26,9 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16X3 = 2
 PREC_BF16 = 3
+PREC_BF16X4 = 4
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3,
-              "bf16": PREC_BF16}
+              "bf16": PREC_BF16, "bf16x4": PREC_BF16X4}
 
 PATH_PLAIN = 0
 PATH_BATCHED = 1
@@ -98,7 +99,7 @@ EXPORTED = [
     "letkf_gpu_get_selection", "letkf_gpu_get_timing",
     "letkf_gpu_set_state_members", "letkf_gpu_set_obs_members",
     "letkf_gpu_get_state_members", "letkf_gpu_autotune", "letkf_gpu_get_tuning",
-    "letkf_gpu_set_tuning", "letkf_gpu_run_graph", "letkf_gpu_get_state_members_noalloc",
+    "letkf_gpu_set_tuning", "letkf_gpu_get_state_members_noalloc", "letkf_gpu_run_graph",
 ]
 
 _FP = C.POINTER(C.c_float)
