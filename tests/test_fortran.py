@@ -45,3 +45,21 @@ def test_fortran_host_matches_oracle(case, tmp_path):
     # the in-place FP32 arrays are the FP64 analysis rounded
     assert np.array_equal(out["state32"], out["state"].astype(np.float32))
     assert np.array_equal(out["mean32"], out["mean"].astype(np.float32))
+
+
+def test_vgauss_vweight_matches_closed_form():
+    """letkf_gpu_loc's vertical LOC_VGAUSS factor (used by the reference-side
+    plugin, fortran/plugins/letkf_loc_vgauss.F90) is exp(-dz^2 / (2 s^2))
+    rounded to REAL: 1, e^-1/2, e^-2 at dz = 0, s, 2s, and the oracle's
+    weight for arbitrary arguments."""
+    import math
+    import subprocess
+    for z, d, s, want in [(0.0, 0.0, 40.0, 1.0), (0.0, 40.0, 40.0, math.exp(-0.5)),
+                          (10.0, 90.0, 40.0, math.exp(-2.0)), (5.0, 35.0, 300.0, None),
+                          (-20.0, 1500.0, 300.0, None)]:
+        r = subprocess.run([FIO.driver(), "--vweight", str(z), str(d), str(s)],
+                           capture_output=True, text=True, check=True)
+        got = float([ln for ln in r.stdout.splitlines() if ln.startswith("vweight")][0].split()[1])
+        if want is None:
+            want = math.exp(-((z - d) ** 2) / (2.0 * s * s))
+        assert got == float(np.float32(want)), (z, d, s, got, want)

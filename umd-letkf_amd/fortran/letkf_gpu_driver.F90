@@ -4,7 +4,12 @@
 !! analysis and the selection diagnostics.
 !!
 !!   letkf_gpu_driver --sizes            C-interop sizes of the ABI structs
+!!   letkf_gpu_driver --vweight z d s    vgauss_vweight(z, d, s) of letkf_gpu_loc
 !!   letkf_gpu_driver <problem> <result>
+!!
+!! LOC_NOVRT and LOC_VGAUSS descriptors are rebuilt from the file's table,
+!! scale and depths through letkf_gpu_loc (to_c_table, vgauss_desc), as a
+!! localizer plugin of the reference builds them.
 !!
 !! Problem file (stream, little endian):
 !!   int64  k, ns, nij, nobs
@@ -28,6 +33,7 @@
 PROGRAM letkf_gpu_driver
   USE, INTRINSIC :: iso_c_binding
   USE letkf_gpu_mod
+  USE letkf_gpu_loc
   USE letkf_solver_gpu
   IMPLICIT NONE
 
@@ -35,7 +41,9 @@ PROGRAM letkf_gpu_driver
   INTEGER(c_int64_t) :: k, ns, nij, nobs
   INTEGER(c_int32_t) :: prec, path, nvars, nlevd, maxg
   REAL(c_float) :: rtps, rtpp, mul
-  TYPE(letkf_loc_desc) :: loc
+  TYPE(letkf_loc_desc) :: loc, loc2
+  CHARACTER(len=64) :: a1, a2, a3
+  REAL(c_float) :: z1, z2, z3
   TYPE(letkf_gpu_dims) :: dims
   TYPE(letkf_linearinterp_lat) :: tab
   TYPE(letkf_gpu_diag) :: diag
@@ -60,6 +68,16 @@ PROGRAM letkf_gpu_driver
           c_sizeof(diag), c_sizeof(timing), c_sizeof(tuning)
      STOP
   END IF
+  IF (TRIM(fin) == "--vweight") THEN
+     CALL get_command_argument(2, a1)
+     CALL get_command_argument(3, a2)
+     CALL get_command_argument(4, a3)
+     READ(a1, *) z1
+     READ(a2, *) z2
+     READ(a3, *) z3
+     PRINT '(A,ES24.16)', "vweight", REAL(vgauss_vweight(z1, z2, z3), c_double)
+     STOP
+  END IF
   CALL get_command_argument(2, fout)
 
   OPEN(newunit=u, file=TRIM(fin), access="stream", form="unformatted", status="old", &
@@ -74,6 +92,18 @@ PROGRAM letkf_gpu_driver
   IF (nlevd > 0) READ(u) lev_depth(1:nlevd)
   loc%lev_depth = c_null_ptr
   IF (nlevd > 0) loc%lev_depth = c_loc(lev_depth)
+  IF (loc%cls == LETKF_LOC_VGAUSS) THEN
+     CALL vgauss_desc(loc%hzloc%lat(1:loc%hzloc%npoints), loc%hzloc%radius(1:loc%hzloc%npoints), &
+          loc%vt_scale, lev_depth(1:nlevd), loc2, ierr)
+     CALL check("vgauss_desc")
+     loc = loc2
+  ELSE IF (loc%cls == LETKF_LOC_NOVRT) THEN
+     loc2%cls = LETKF_LOC_NOVRT
+     CALL to_c_table(loc%hzloc%lat(1:loc%hzloc%npoints), loc%hzloc%radius(1:loc%hzloc%npoints), &
+          loc2%hzloc, ierr)
+     CALL check("to_c_table")
+     loc = loc2
+  END IF
   ALLOCATE(state(k, ns, nij), mean(ns, nij), sprd(ns, nij), lat(nij), lon(nij), imask(nij))
   READ(u) state, mean, sprd, lat, lon, imask
   ALLOCATE(mask(nij))

@@ -18,7 +18,7 @@ MODULE letkf_gpu_mod
        LETKF_ECAPACITY = 5, LETKF_ESTATE = 6
   ! arithmetic and execution path
   INTEGER(c_int32_t), PARAMETER, PUBLIC :: LETKF_PREC_FP64 = 0, LETKF_PREC_FP32 = 1, &
-       LETKF_PREC_BF16X3 = 2, LETKF_PREC_BF16 = 3
+       LETKF_PREC_BF16X3 = 2, LETKF_PREC_BF16 = 3, LETKF_PREC_BF16X4 = 4
   INTEGER(c_int32_t), PARAMETER, PUBLIC :: LETKF_PATH_PLAIN = 0, LETKF_PATH_BATCHED = 1
   ! localizer classes (localization.class)
   INTEGER(c_int32_t), PARAMETER, PUBLIC :: LETKF_LOC_NOVRT = 1, LETKF_LOC_OCEAN = 2, &

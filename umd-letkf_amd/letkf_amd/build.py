@@ -97,11 +97,15 @@ def amdflang() -> str:
 
 
 def build_fortran(verbose: bool = True) -> str:
-    """Compile letkf_gpu_mod, letkf_solver_gpu and letkf_gpu_driver; the
-    driver links libletkf_gpu.so through an $ORIGIN-relative rpath."""
+    """Compile letkf_gpu_mod, letkf_gpu_loc, letkf_solver_gpu and
+    letkf_gpu_driver; the driver links libletkf_gpu.so through an
+    $ORIGIN-relative rpath.  (fortran/plugins/ holds a localizer plugin for
+    the reference's own build; it uses the reference's modules and is not
+    compiled here.)"""
     os.makedirs(FBUILD, exist_ok=True)
     objs = []
-    for src in ("letkf_gpu_mod.F90", "letkf_solver_gpu.F90", "letkf_gpu_driver.F90"):
+    for src in ("letkf_gpu_mod.F90", "letkf_gpu_loc.F90", "letkf_solver_gpu.F90",
+                "letkf_gpu_driver.F90"):
         obj = os.path.join(FBUILD, src.replace(".F90", ".o"))
         cmd = [amdflang(), "-O2", "-c", os.path.join(FORTRAN, src), "-module-dir", FBUILD,
                "-I", FBUILD, "-o", obj]
