@@ -4,7 +4,10 @@
 Workload (default): the 0.25-degree 1440x720 surface analysis with k = 50,
 2M observations on 30 synthetic swath arcs and a 300 km cut-off radius
 (seeded synthetic data, inputs resident in HBM before the timed region).
-A step is one full analysis (letkf_gpu_run) over all 1,036,800 gridpoints.
+A step is one full analysis (letkf_gpu_run) over all 1,036,800 gridpoints,
+on the batched path with split-BF16 products of all four terms and
+compensated accumulation (bf16x4); the three-term split (bf16x3) is timed
+beside it on the same inputs.
 
 Printed: one JSON line with the metric, the roofline of the product kernel
 (the largest single kernel; from the library's HIP events on its own
@@ -238,7 +241,7 @@ def cpu_baseline(prob, solver, seconds, nthreads):
            "perturbations_rel": rel(st[:, :, pts] - mn[None, :, pts], pert_o),
            "increment_rel": rel(mn[:, pts] - prob.mean[:, pts], inc_o),
            "mean_maxabs": float(np.abs(mn[:, pts] - orc["mean"]).max())}
-    return base, err
+    return base, err, (pts, orc)
 
 
 def bench_cycles(args):
@@ -463,7 +466,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3")
     ap.add_argument("--path", default="batched", choices=["plain", "batched"])
-    ap.add_argument("--precision", default="bf16x3", choices=list(PEAK))
+    ap.add_argument("--precision", default="bf16x4", choices=list(PEAK))
+    ap.add_argument("--no-alternatives", action="store_true",
+                    help="skip timing the other split-BF16 product (bf16x3 / bf16x4) beside it")
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="skip timing the plain FP64 path on the same workload")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -576,9 +581,43 @@ def main():
 
     if not args.no_cpu:
         nt = cpu_threads()
-        out["cpu_baseline"], out["error_vs_fp64_oracle"] = cpu_baseline(
+        out["cpu_baseline"], out["error_vs_fp64_oracle"], sample = cpu_baseline(
             prob, solver, args.cpu_seconds, nt)
     solver.final()
+    if args.path == "batched" and not args.no_alternatives:
+        # the other split-BF16 product on the same inputs: time and error
+        alt = {}
+        for prec in ("bf16x3", "bf16x4"):
+            if prec == args.precision:
+                continue
+            s3 = LetkfGpuSolver(prob, precision=prec, path="batched")
+            if tun_fixed:
+                s3.set_tuning(**tun_fixed)
+            elif not args.no_autotune:
+                s3.autotune(repeats=1)
+            s3.run()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                s3.run()
+            dt = (time.perf_counter() - t0) / args.steps
+            rec = {"ms_per_step": round(1e3 * dt, 3), "gridpoints_per_s": round(prob.nij / dt, 1),
+                   "product_ms": round(s3.timing()["product_ms"], 3)}
+            if not args.no_cpu:
+                pts, orc = sample
+                st, mn, sp = s3.state_f64()
+
+                def rel(a, b):
+                    den = np.abs(b).max()
+                    return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
+                rec["error_vs_fp64_oracle"] = {
+                    "mean_rel": rel(mn[:, pts], orc["mean"]), "sprd_rel": rel(sp[:, pts], orc["sprd"]),
+                    "members_rel": rel(st[:, :, pts], orc["state"]),
+                    "perturbations_rel": rel(st[:, :, pts] - mn[None, :, pts],
+                                             orc["state"] - orc["mean"][None])}
+            s3.final()
+            alt[prec] = rec
+            log(f"[bench] {prec}: {rec}")
+        out["other_precisions"] = alt
     if not args.no_ingest:
         # the steps either side of the path on the device: members in file
         # order -> perturbations / mean / spread, members' hx + qc -> hx

@@ -15,6 +15,12 @@ from letkf_amd import synth
 pytestmark = pytest.mark.gpu
 
 TOL = {"fp32": 1e-5, "bf16x3": 1e-5, "bf16x4": 1e-5, "bf16": 1e-2}
+# members (full values) and perturbations (members minus mean), against
+# the oracle's: the compensated products (fp32, bf16x4) meet 1e-5 on the
+# members; bf16x3 drops the lo*lo term (positive on the diagonal) and keeps
+# plain FP32 accumulation, so its bound is looser (stated, asserted)
+TOL_MEMBERS = {"fp32": 1e-5, "bf16x3": 1e-4, "bf16x4": 1e-5, "bf16": 1e-2}
+TOL_PERT = {"fp32": 5e-5, "bf16x3": 2e-3, "bf16x4": 5e-5, "bf16": 1e-1}
 
 
 def _mixed(prob, prec, points=None):
@@ -25,9 +31,12 @@ def _mixed(prob, prec, points=None):
     errs = dict(mean=rel(gpu["mean"][:, pts], orc["mean"]),
                 sprd=rel(gpu["sprd"][:, pts], orc["sprd"]),
                 members=rel(gpu["state"][:, :, pts], orc["state"]),
+                pert=rel(gpu["state"][:, :, pts] - gpu["mean"][None, :, pts],
+                         orc["state"] - orc["mean"][None]),
                 inc=rel(gpu["mean"][:, pts] - prob.mean[:, pts], orc["mean"] - prob.mean[:, pts]))
     print(prec, prob.meta, errs, flush=True)
     assert errs["mean"] <= TOL[prec] and errs["sprd"] <= TOL[prec], errs
+    assert errs["members"] <= TOL_MEMBERS[prec] and errs["pert"] <= TOL_PERT[prec], errs
     return errs
 
 

@@ -22,15 +22,16 @@
 //                of the dropped lo*lo term, which is positive on the diagonal
 //
 // Accumulation.  The MFMAs accumulate in FP32.  For PREC_FP32 and
-// PREC_BF16X4 the accumulators are compensated: every 256 observations the
-// partial sums are added into FP32 running sums S by the error-free TwoSum
-// (S + a = s + e exactly), the rounding errors e collected in a per-wave
-// carry array in LDS, and the partial sums restart from zero.  The result
-// S + carry + partial is formed in FP64.  The error of an m-observation
-// product then grows like 256 sqrt(m) rounding units instead of m^1.5 / 2:
-// the member perturbations of gridpoints with thousands of strongly
-// correlated observations move from ~1e-4 to ~5e-6 relative to the FP64
-// analysis.  PREC_BF16X3 and PREC_BF16 keep plain FP32 accumulation.
+// PREC_BF16X4 the accumulators are compensated: every F observations (the
+// fold period, a power of two >= 32) the partial sums are added into FP32
+// running sums S by the error-free TwoSum (S + a = s + e exactly), the
+// rounding errors e collected in a per-wave carry array in LDS, and the
+// partial sums restart from zero.  The result S + carry + partial is formed
+// in FP64.  The error of an m-observation product then grows like
+// F sqrt(m) rounding units instead of m^1.5 / 2 (products of FP32 operands
+// summed exactly would be ~1e-9 from the FP64 analysis; the FP32 partial
+// sums are what is left).  PREC_BF16X3 and PREC_BF16 keep plain FP32
+// accumulation.
 //
 // The FP32 result is widened to FP64 for the transform.
 #include <hip/hip_runtime.h>
@@ -198,7 +199,7 @@ template <int KP, int PREC>
 __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
                                                    const int32_t* __restrict__ perm,
                                                    double* __restrict__ B,
-                                                   double* __restrict__ bvec) {
+                                                   double* __restrict__ bvec, int fold_mask) {
   constexpr int NB = KP / 32;
   constexpr int NBLK = NB * (NB + 1) / 2;
   const int64_t pair = perm[C.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x];
@@ -255,8 +256,8 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
     __builtin_amdgcn_sched_barrier(0);
     mx_step<NB, PREC>(acc, fb);
     __builtin_amdgcn_sched_barrier(0);
-    // every 256 observations (wave-uniform)
-    if (COMP && ((j0 + 32) & 255) == 0) mx_fold<NBLK>(acc, S, carry, lane);
+    // every fold_mask + 1 observations (wave-uniform)
+    if (COMP && ((j0 + 32) & fold_mask) == 0) mx_fold<NBLK>(acc, S, carry, lane);
   }
 
   // 32x32 f32 accumulator: register r of lane l holds row 8(r/4) + 4(l/32) + r%4
@@ -286,29 +287,32 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
 
 template <int KP>
 static void launch_kp(hipStream_t st, int prec, unsigned nb, const ObsDev& O, int k,
-                      const ChunkDev& C, const int32_t* perm, double* B, double* bvec) {
+                      const ChunkDev& C, const int32_t* perm, double* B, double* bvec,
+                      int fm) {
   if (prec == LETKF_PREC_FP32)
-    k_product_mx<KP, LETKF_PREC_FP32><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec);
+    k_product_mx<KP, LETKF_PREC_FP32><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec, fm);
   else if (prec == LETKF_PREC_BF16)
-    k_product_mx<KP, LETKF_PREC_BF16><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec);
+    k_product_mx<KP, LETKF_PREC_BF16><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec, fm);
   else if (prec == LETKF_PREC_BF16X4)
-    k_product_mx<KP, LETKF_PREC_BF16X4><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec);
+    k_product_mx<KP, LETKF_PREC_BF16X4><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec, fm);
   else
-    k_product_mx<KP, LETKF_PREC_BF16X3><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec);
+    k_product_mx<KP, LETKF_PREC_BF16X3><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec, fm);
 }
 
 void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev& S,
-                       const ChunkDev& C, const int32_t* perm, double* B, double* bvec) {
+                       const ChunkDev& C, const int32_t* perm, double* B, double* bvec,
+                       int fold_log2) {
   const int64_t np = C.ngp * C.G;
   if (np <= 0) return;
   const int k = (int)S.k;
   const int kp = ((k + 1 + 31) / 32) * 32;
   const unsigned nb = (unsigned)np;
+  const int fm = (1 << (fold_log2 < 5 ? 5 : fold_log2)) - 1;   // the loop steps by 32
   switch (kp) {
-    case 32: launch_kp<32>(st, prec, nb, O, k, C, perm, B, bvec); break;
-    case 64: launch_kp<64>(st, prec, nb, O, k, C, perm, B, bvec); break;
-    case 96: launch_kp<96>(st, prec, nb, O, k, C, perm, B, bvec); break;
-    default: launch_kp<128>(st, prec, nb, O, k, C, perm, B, bvec); break;
+    case 32: launch_kp<32>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
+    case 64: launch_kp<64>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
+    case 96: launch_kp<96>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
+    default: launch_kp<128>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
   }
 }
 

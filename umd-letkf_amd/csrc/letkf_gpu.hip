@@ -143,6 +143,7 @@ struct letkf_gpu_t {
   size_t live_bytes = 0;
   size_t mem_limit = 0;   // 0 = none
   int tune_xcd = 0;       // XCD-contiguous block order in the product kernels
+  int fold_log2 = 7;      // compensated products: partial sums folded every 2^n obs
   int tune_bucket = 2;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
   int tune_bbits = 0;     // bucket edges: powers of two split in 2^bbits parts
   int tune_trireg = 1;    // register-resident tridiagonalization (32 <= k < 64)
@@ -294,6 +295,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   h->nij = d->ij_count;
   if (const char* e = getenv("LETKF_GPU_MEM_LIMIT")) h->mem_limit = strtoull(e, nullptr, 10);
   if (const char* e = getenv("LETKF_GPU_XCD")) h->tune_xcd = atoi(e);
+  if (const char* e = getenv("LETKF_GPU_FOLD")) h->fold_log2 = std::min(10, std::max(5, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = std::min(3, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_BBITS")) h->tune_bbits = std::min(2, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_TRIREG")) h->tune_trireg = atoi(e) != 0;
@@ -1139,7 +1141,8 @@ static int run_eager(letkf_gpu_t* h) {
       RUNCHK(hipcub::DeviceRadixSort::SortPairs(h->d_stmp, sb, h->d_bkey, h->d_bkey2,
                                                 h->d_bval, h->d_perm, (int)np, 0, 32, st));
       RUNCHK(hipEventRecord(ce[3], st));
-      launch_product_mx(st, h->dims.precision, O, S, C, h->d_perm, h->d_B, h->d_b);
+      launch_product_mx(st, h->dims.precision, O, S, C, h->d_perm, h->d_B, h->d_b,
+                        h->fold_log2);
     } else {
       RUNCHK(hipEventRecord(ce[3], st));
       launch_product_f64(st, O, S, C, h->d_B, h->d_b);
@@ -1220,7 +1223,8 @@ static hipError_t enqueue_analysis(letkf_gpu_t* h) {
       e = hipcub::DeviceRadixSort::SortPairs(h->d_stmp, sb, h->d_bkey, h->d_bkey2, h->d_bval,
                                              h->d_perm, (int)np, 0, 32, st);
       if (e != hipSuccess) break;
-      launch_product_mx(st, h->dims.precision, W.O, W.S, C, h->d_perm, h->d_B, h->d_b);
+      launch_product_mx(st, h->dims.precision, W.O, W.S, C, h->d_perm, h->d_B, h->d_b,
+                        h->fold_log2);
     } else {
       launch_product_f64(st, W.O, W.S, C, h->d_B, h->d_b);
     }

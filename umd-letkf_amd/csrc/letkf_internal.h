@@ -187,7 +187,8 @@ hipError_t launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDe
                                   const double* B, const double* bvec, const int* only_flagged,
                                   double* ana);
 void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev& S,
-                       const ChunkDev& C, const int32_t* perm, double* B, double* bvec);
+                       const ChunkDev& C, const int32_t* perm, double* B, double* bvec,
+                       int fold_log2);
 void launch_bucket_keys(hipStream_t st, int mode, int bbits, int64_t np, int G, int64_t i0,
                         const int64_t* cnt, const uint8_t* solve, const float* lat,
                         const float* lon, uint32_t* keys, int32_t* vals);
