@@ -68,7 +68,8 @@ def product_kernel(path, precision):
 
 
 def tuning_key(tun):
-    return "order{order}_bb{bucket_bits}".format(**tun) if tun else "default"
+    return ("order{order}_bb{bucket_bits}_xcd{xcd}".format(**{"xcd": 2, **tun})
+            if tun else "default")
 
 
 def roofline(t, k, path, precision, workload, tun=None, slabs_per_problem=1.0, ns=1,
@@ -482,7 +483,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch choices instead of letkf_gpu_autotune")
     ap.add_argument("--tuning", default=None,
-                    help="launch choices order,bucket_bits[,tridiag_reg,overlap,chunk_log2] set "
+                    help="launch choices order,bucket_bits[,tridiag_reg,overlap,chunk_log2,xcd] set "
                          "instead of autotuning (as a saved autotune result is reused)")
     ap.add_argument("--cycles", type=int, default=10, help="c5: number of analysis cycles")
     ap.add_argument("--no-graph", action="store_true",
@@ -494,9 +495,9 @@ def main():
     tun_fixed = None
     if args.tuning:
         v = [int(x) for x in args.tuning.split(",")]
-        v += [1, 0, 20][len(v) - 2:] if len(v) < 5 else []
+        v += [1, 0, 20, 2][len(v) - 2:] if len(v) < 6 else []
         tun_fixed = dict(order=v[0], bucket_bits=v[1], tridiag_reg=v[2], overlap=v[3],
-                         chunk_log2=v[4])
+                         chunk_log2=v[4], xcd=v[5])
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, args.config, args.path, args.precision,
                       args.problems_per_dispatch, tun_fixed)

@@ -148,7 +148,10 @@ typedef struct letkf_gpu_timing {
  * 32 <= k < 64, 0 the LDS kernel; overlap: 1 searches the next chunk of
  * gridpoints on a second stream while the current one is solved;
  * chunk_log2: at most 2^chunk_log2 (gridpoint, group) problems per chunk
- * (17..20).  None of them changes the analysis. */
+ * (17..20); xcd: how the product kernel's workgroups meet the 8 XCDs (each
+ * with its own L2), 0 in launch order, 1..3 each XCD runs of 32, 64 or 128
+ * consecutive problems, the runs of the XCDs interleaved (default 2).  None
+ * of them changes the analysis. */
 typedef struct letkf_gpu_tuning {
   int32_t order;
   int32_t bucket_bits;
@@ -158,6 +161,7 @@ typedef struct letkf_gpu_tuning {
   double default_ms;   /* autotune: analysis time of the defaults     */
   int32_t overlap;
   int32_t chunk_log2;
+  int32_t xcd;
 } letkf_gpu_tuning;
 
 typedef struct letkf_gpu_t letkf_gpu_t;

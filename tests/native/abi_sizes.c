@@ -24,5 +24,6 @@ int main(void) {
   F(letkf_gpu_tuning, n_tried);
   F(letkf_gpu_tuning, default_ms);
   F(letkf_gpu_tuning, chunk_log2);
+  F(letkf_gpu_tuning, xcd);
   return 0;
 }

@@ -81,8 +81,9 @@ def test_rational_and_eigen_paths_agree(prec, k, ns, monkeypatch):
 
 def test_launch_choices_and_autotune():
     """The problem order (four modes, bucket edges split 1, 2 or 4 ways per
-    octave) only reorders independent problems: the analysis is bitwise the
-    same under each.  The register and LDS tridiagonalizations agree to
+    octave) and the XCD placement (launch order, runs of 32, 64 or 128) only
+    reorder independent problems: the analysis is bitwise the same under
+    each.  The register and LDS tridiagonalizations agree to
     rounding.  autotune times them, keeps the fastest and leaves that
     analysis; bad settings are refused."""
     from letkf_amd.solver import LetkfError, LetkfGpuSolver
@@ -92,23 +93,24 @@ def test_launch_choices_and_autotune():
         st0, mn0, sp0 = s.state_f64()
         for order in range(4):
             for bb in (range(3) if order >= 2 else [0]):
-                print("order", order, "bucket_bits", bb, flush=True)
-                s.set_tuning(order, bb, 1)
-                s.run()
-                st, mn, sp = s.state_f64()
-                assert np.array_equal(st, st0) and np.array_equal(mn, mn0)
+                for xcd in (0, 1, 2, 3):
+                    print("order", order, "bucket_bits", bb, "xcd", xcd, flush=True)
+                    s.set_tuning(order, bb, 1, xcd=xcd)
+                    s.run()
+                    st, mn, sp = s.state_f64()
+                    assert np.array_equal(st, st0) and np.array_equal(mn, mn0)
         s.set_tuning(2, 0, 0)
         s.run()
         st, mn, sp = s.state_f64()
         assert rel(st, st0) <= 1e-12 and rel(mn, mn0) <= 1e-12
         t = s.autotune(repeats=1)
         print("autotune", t, flush=True)
-        assert t["n_tried"] == 9 and t["best_ms"] <= t["default_ms"]
+        assert t["n_tried"] == 12 and t["best_ms"] <= t["default_ms"]
         assert s.tuning()["order"] == t["order"]
         st, mn, sp = s.state_f64()
         assert rel(st, st0) <= 1e-12 and rel(mn, mn0) <= 1e-12
         for bad in ((4, 0, 1), (2, 3, 1), (2, 0, 2), (2, 0, 1, 2), (2, 0, 1, 1, 16),
-                    (2, 0, 1, 1, 21)):
+                    (2, 0, 1, 1, 21), (2, 0, 1, 1, 20, 4), (2, 0, 1, 1, 20, -1)):
             with pytest.raises(LetkfError):
                 s.set_tuning(*bad)
 

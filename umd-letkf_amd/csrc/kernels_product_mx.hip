@@ -202,7 +202,7 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
                                                    double* __restrict__ bvec, int fold_mask) {
   constexpr int NB = KP / 32;
   constexpr int NBLK = NB * (NB + 1) / 2;
-  const int64_t pair = perm[C.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x];
+  const int64_t pair = perm[xcd_order(C.xcd, blockIdx.x, gridDim.x)];
   if (!C.solve[pair]) return;
   const int m = (int)C.cnt[pair];
   const int64_t off = C.off[pair];

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_product_f64(ObsDev O, int k, ChunkDev C
   constexpr int LD = KP + 16;
   __shared__ double Ys[64 * LD];
   __shared__ double sc[64];
-  const int64_t pair = C.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+  const int64_t pair = xcd_order(C.xcd, blockIdx.x, gridDim.x);
   if (!C.solve[pair]) return;
   const int64_t m = C.cnt[pair];
   const int64_t off = C.off[pair];

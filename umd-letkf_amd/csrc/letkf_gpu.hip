@@ -142,7 +142,7 @@ struct letkf_gpu_t {
   std::unordered_map<void*, size_t> live;
   size_t live_bytes = 0;
   size_t mem_limit = 0;   // 0 = none
-  int tune_xcd = 0;       // XCD-contiguous block order in the product kernels
+  int tune_xcd = 2;       // product kernels: runs of 64 problems per XCD
   int fold_log2 = 7;      // compensated products: partial sums folded every 2^n obs
   int tune_bucket = 2;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
   int tune_bbits = 0;     // bucket edges: powers of two split in 2^bbits parts
@@ -294,7 +294,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   h->ns = d->grid_ns;
   h->nij = d->ij_count;
   if (const char* e = getenv("LETKF_GPU_MEM_LIMIT")) h->mem_limit = strtoull(e, nullptr, 10);
-  if (const char* e = getenv("LETKF_GPU_XCD")) h->tune_xcd = atoi(e);
+  if (const char* e = getenv("LETKF_GPU_XCD")) h->tune_xcd = std::min(3, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_FOLD")) h->fold_log2 = std::min(10, std::max(5, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = std::min(3, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_BBITS")) h->tune_bbits = std::min(2, std::max(0, atoi(e)));
@@ -1569,7 +1569,7 @@ int letkf_gpu_get_timing(const letkf_gpu_t* h, letkf_gpu_timing* t) {
 int letkf_gpu_get_tuning(const letkf_gpu_t* h, letkf_gpu_tuning* out) {
   if (!h || !out) return LETKF_EINVAL;
   *out = letkf_gpu_tuning{h->tune_bucket, h->tune_bbits, h->tune_trireg, 0, 0.0, 0.0,
-                          h->tune_overlap, h->tune_chunk_log2};
+                          h->tune_overlap, h->tune_chunk_log2, h->tune_xcd};
   return LETKF_OK;
 }
 
@@ -1585,12 +1585,15 @@ int letkf_gpu_set_tuning(letkf_gpu_t* h, const letkf_gpu_tuning* t) {
     return fail(h, LETKF_EINVAL, "tuning overlap %d not 0 or 1", t->overlap);
   if (t->chunk_log2 < 17 || t->chunk_log2 > 20)
     return fail(h, LETKF_EINVAL, "tuning chunk_log2 %d outside 17..20", t->chunk_log2);
+  if (t->xcd < 0 || t->xcd > 3)
+    return fail(h, LETKF_EINVAL, "tuning xcd %d outside 0..3", t->xcd);
   ++h->gen;
   h->tune_bucket = t->order;
   h->tune_bbits = t->bucket_bits;
   h->tune_trireg = t->tridiag_reg;
   h->tune_overlap = t->overlap;
   h->tune_chunk_log2 = t->chunk_log2;
+  h->tune_xcd = t->xcd;
   return LETKF_OK;
 }
 
@@ -1644,6 +1647,17 @@ int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out) {
         t.bucket_bits = bb;
         if ((rc = consider(t))) return rc;
       }
+  }
+  // the product kernel's XCD placement (batched path: that kernel reads the
+  // observation columns neighbouring problems share)
+  if (h->batched()) {
+    const letkf_gpu_tuning base = best;
+    for (int x = 0; x <= 3; ++x) {
+      if (x == base.xcd) continue;
+      letkf_gpu_tuning t = base;
+      t.xcd = x;
+      if ((rc = consider(t))) return rc;
+    }
   }
   if (h->k >= 32 && h->k < 64 && h->eigK == 64) {
     letkf_gpu_tuning t = best;
