@@ -144,8 +144,9 @@ typedef struct letkf_gpu_timing {
  * product, 0 gridpoint order, 1 longest first, 2 buckets of the kept
  * count largest first, 3 buckets with a Morton walk inside; bucket_bits:
  * the bucket edges are the powers of two split into 2^bucket_bits parts
- * (0..2); tridiag_reg: 1 register-resident Householder reduction for
- * 32 <= k < 64, 0 the LDS kernel; overlap: 1 searches the next chunk of
+ * (0..2); tridiag_reg: the Householder reduction for 32 <= k < 64, 1 one
+ * problem per wave with the matrix in registers, 2 two problems per wave
+ * (k <= 50; 1 above), 0 the LDS kernel; overlap: 1 searches the next chunk of
  * gridpoints on a second stream while the current one is solved;
  * chunk_log2: at most 2^chunk_log2 (gridpoint, group) problems per chunk
  * (17..20); xcd: how the product kernel's workgroups meet the 8 XCDs (each

@@ -83,8 +83,8 @@ def test_launch_choices_and_autotune():
     """The problem order (four modes, bucket edges split 1, 2 or 4 ways per
     octave) and the XCD placement (launch order, runs of 32, 64 or 128) only
     reorder independent problems: the analysis is bitwise the same under
-    each.  The register and LDS tridiagonalizations agree to
-    rounding.  autotune times them, keeps the fastest and leaves that
+    each.  The register (one and two problems per wave) and LDS
+    tridiagonalizations agree to rounding.  autotune times them, keeps the fastest and leaves that
     analysis; bad settings are refused."""
     from letkf_amd.solver import LetkfError, LetkfGpuSolver
     p = synth.random_problem(seed=777, k=40, ns=2, nobs=4000, rtps=0.4)
@@ -99,17 +99,18 @@ def test_launch_choices_and_autotune():
                     s.run()
                     st, mn, sp = s.state_f64()
                     assert np.array_equal(st, st0) and np.array_equal(mn, mn0)
-        s.set_tuning(2, 0, 0)
-        s.run()
-        st, mn, sp = s.state_f64()
-        assert rel(st, st0) <= 1e-12 and rel(mn, mn0) <= 1e-12
+        for tr in (0, 2):
+            s.set_tuning(2, 0, tr)
+            s.run()
+            st, mn, sp = s.state_f64()
+            assert rel(st, st0) <= 1e-12 and rel(mn, mn0) <= 1e-12, tr
         t = s.autotune(repeats=1)
         print("autotune", t, flush=True)
-        assert t["n_tried"] == 12 and t["best_ms"] <= t["default_ms"]
+        assert t["n_tried"] == 13 and t["best_ms"] <= t["default_ms"]
         assert s.tuning()["order"] == t["order"]
         st, mn, sp = s.state_f64()
         assert rel(st, st0) <= 1e-12 and rel(mn, mn0) <= 1e-12
-        for bad in ((4, 0, 1), (2, 3, 1), (2, 0, 2), (2, 0, 1, 2), (2, 0, 1, 1, 16),
+        for bad in ((4, 0, 1), (2, 3, 1), (2, 0, 3), (2, 0, 1, 2), (2, 0, 1, 1, 16),
                     (2, 0, 1, 1, 21), (2, 0, 1, 1, 20, 4), (2, 0, 1, 1, 20, -1)):
             with pytest.raises(LetkfError):
                 s.set_tuning(*bad)

@@ -735,12 +735,16 @@ static void tridiag_launch(hipStream_t st, unsigned nb, size_t lds, int k, const
                            const StateDev& S, const VarsDev& V, const ChunkDev& C,
                            const DiagDev& D, double* B, const double* bvec, double* tau,
                            double* dd, double* ee, double* U, int* nvec, int* status,
-                           bool reg) {
-  // k >= 32 (K = 64): the register-resident reduction, rows padded to a
-  // multiple of 8 (to 50 for k = 49 and 50), unless reg is false (the LDS
-  // kernel)
+                           int reg, const int32_t* perm) {
+  // k >= 32 (K = 64), rows padded to a multiple of 8 (to 50 for k = 49 and
+  // 50): reg 2 the two-problem register kernel (k <= 50), reg 1 (or 2 past
+  // k = 50) the one-problem register kernel, reg 0 the LDS kernel
   const int kq = (k == 49 || k == 50) ? 50 : (k + 7) / 8 * 8;
-  if (K == 64 && reg &&
+  if (K == 64 && reg == 2 &&
+      launch_tridiag_pair(st, (int64_t)nb, perm, kq, NV, k, K, L, S, V, C, D, B, bvec, tau, dd,
+                          ee, U, nvec, status))
+    return;
+  if (K == 64 && reg != 0 &&
       launch_tridiag_reg(st, nb, kq, NV, k, K, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec,
                          status))
     return;
@@ -759,18 +763,19 @@ static void pipeline(hipStream_t st, const LocDev& L, const StateDev& S, const V
                      const ChunkDev& C, const DiagDev& D, double* B, const double* bvec,
                      double* tau, double* dd, double* ee, double2* rot, int cap_rot,
                      int4* itr, int cap_it, int* nit, int* status, double* Z, int NV,
-                     double* U, int* nvec, const RatTab& R, double* ana, bool reg) {
+                     double* U, int* nvec, const RatTab& R, double* ana, int reg,
+                     const int32_t* perm) {
   const int64_t np = C.ngp * C.G;
   const int k = (int)S.k;
   const unsigned nb = (unsigned)np;
   const unsigned nl = (unsigned)((np + 63) / 64);
   const size_t lds = sizeof(double) * (size_t)k * (k + 1);
   switch (NV) {
-    case 0: tridiag_launch<K, 0>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg); break;
-    case 2: tridiag_launch<K, 2>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg); break;
-    case 3: tridiag_launch<K, 3>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg); break;
-    case 5: tridiag_launch<K, 5>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg); break;
-    default: tridiag_launch<K, 9>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg); break;
+    case 0: tridiag_launch<K, 0>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm); break;
+    case 2: tridiag_launch<K, 2>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm); break;
+    case 3: tridiag_launch<K, 3>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm); break;
+    case 5: tridiag_launch<K, 5>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm); break;
+    default: tridiag_launch<K, 9>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm); break;
   }
   if (NV > 0 && R.nb > 0) {
     // rational T^(-1/2): no QL, no eigenvectors
@@ -813,14 +818,15 @@ void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
                            double* B, const double* bvec, double* tau, double* dd,
                            double* ee, double2* rot, int cap_rot, int4* itr,
                            int cap_it, int* nit, int* status, double* Z, int NV, double* U,
-                           int* nvec, const RatTab& R, double* ana, bool trireg) {
+                           int* nvec, const RatTab& R, double* ana, int trireg,
+                           const int32_t* perm) {
   if (C.ngp * C.G <= 0) return;
   if (eigen_pipeline_K((int)S.k) == 32)
     pipeline<32>(st, L, S, V, C, D, B, bvec, tau, dd, ee, rot, cap_rot, itr, cap_it, nit,
-                 status, Z, NV, U, nvec, R, ana, trireg);
+                 status, Z, NV, U, nvec, R, ana, trireg, perm);
   else
     pipeline<64>(st, L, S, V, C, D, B, bvec, tau, dd, ee, rot, cap_rot, itr, cap_it, nit,
-                 status, Z, NV, U, nvec, R, ana, trireg);
+                 status, Z, NV, U, nvec, R, ana, trireg, perm);
 }
 
 }  // namespace lk

@@ -146,7 +146,8 @@ struct letkf_gpu_t {
   int fold_log2 = 7;      // compensated products: partial sums folded every 2^n obs
   int tune_bucket = 2;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
   int tune_bbits = 0;     // bucket edges: powers of two split in 2^bbits parts
-  int tune_trireg = 1;    // register-resident tridiagonalization (32 <= k < 64)
+  int tune_trireg = 1;    // tridiagonalization (32 <= k < 64): 0 LDS, 1 register,
+                          // 2 two problems per wave (k <= 50)
   int tune_overlap = 0;   // search of the next chunk concurrent with the solve
   int tune_chunk_log2 = 20;   // at most 2^n problems per chunk (17..20)
   int tune_vec = 1;       // vector path when every group has few slabs
@@ -298,7 +299,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   if (const char* e = getenv("LETKF_GPU_FOLD")) h->fold_log2 = std::min(10, std::max(5, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = std::min(3, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_BBITS")) h->tune_bbits = std::min(2, std::max(0, atoi(e)));
-  if (const char* e = getenv("LETKF_GPU_TRIREG")) h->tune_trireg = atoi(e) != 0;
+  if (const char* e = getenv("LETKF_GPU_TRIREG")) h->tune_trireg = std::min(2, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_VEC")) h->tune_vec = atoi(e);
   if (const char* e = getenv("LETKF_GPU_RAT")) h->tune_rat = atoi(e);
   if (const char* e = getenv("LETKF_GPU_MAT")) h->tune_mat = atoi(e);
@@ -988,7 +989,7 @@ static hipError_t enqueue_solve(letkf_gpu_t* h, hipStream_t st, const RunViews& 
     launch_eigen_pipeline(st, h->loc, W.S, W.V, C, W.D, h->d_B, h->d_b, h->d_tau, h->d_dd,
                           h->d_ee, h->d_rot, h->cap_rot, h->d_itr, h->cap_it, h->d_nit,
                           h->d_status, h->d_Z, h->vec_nv, h->d_U, h->d_nvec, R, h->d_ana,
-                          h->tune_trireg != 0);
+                          h->tune_trireg, h->batched() ? h->d_perm : nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // problems the QL could not finish go to the Jacobi solver
@@ -1579,8 +1580,8 @@ int letkf_gpu_set_tuning(letkf_gpu_t* h, const letkf_gpu_tuning* t) {
   if (t->order < 0 || t->order > 3) return fail(h, LETKF_EINVAL, "tuning order %d outside 0..3", t->order);
   if (t->bucket_bits < 0 || t->bucket_bits > 2)
     return fail(h, LETKF_EINVAL, "tuning bucket_bits %d outside 0..2", t->bucket_bits);
-  if (t->tridiag_reg != 0 && t->tridiag_reg != 1)
-    return fail(h, LETKF_EINVAL, "tuning tridiag_reg %d not 0 or 1", t->tridiag_reg);
+  if (t->tridiag_reg < 0 || t->tridiag_reg > 2)
+    return fail(h, LETKF_EINVAL, "tuning tridiag_reg %d outside 0..2", t->tridiag_reg);
   if (t->overlap != 0 && t->overlap != 1)
     return fail(h, LETKF_EINVAL, "tuning overlap %d not 0 or 1", t->overlap);
   if (t->chunk_log2 < 17 || t->chunk_log2 > 20)
@@ -1660,9 +1661,13 @@ int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out) {
     }
   }
   if (h->k >= 32 && h->k < 64 && h->eigK == 64) {
-    letkf_gpu_tuning t = best;
-    t.tridiag_reg = 1 - t.tridiag_reg;
-    if ((rc = consider(t))) return rc;
+    const letkf_gpu_tuning base = best;
+    for (int tr = 0; tr <= (h->k <= 50 ? 2 : 1); ++tr) {
+      if (tr == base.tridiag_reg) continue;
+      letkf_gpu_tuning t = base;
+      t.tridiag_reg = tr;
+      if ((rc = consider(t))) return rc;
+    }
   }
   // overlap and chunk size matter only when the smallest chunk size would
   // split the analysis

@@ -37,11 +37,13 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-# per-source extra flags: the register-resident tridiagonalization needs its
-# step loop fully unrolled (static register indices), beyond LLVM's default
+# per-source extra flags: the register-resident tridiagonalizations need their
+# step loops fully unrolled (static register indices), beyond LLVM's default
 # size limit for unroll pragmas
 EXTRA = {f"tridiag_reg_kq{kq}.hip": ["-mllvm", "-pragma-unroll-threshold=2000000"]
          for kq in (32, 40, 48, 50, 56, 64)}
+EXTRA.update({f"tridiag_pair_kq{kq}.hip": ["-mllvm", "-pragma-unroll-threshold=2000000"]
+              for kq in (32, 40, 48, 50)})
 
 
 def _compile(src: str) -> str:
