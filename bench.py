@@ -68,8 +68,10 @@ def product_kernel(path, precision):
 
 
 def tuning_key(tun):
-    return ("order{order}_bb{bucket_bits}_xcd{xcd}".format(**{"xcd": 2, **tun})
-            if tun else "default")
+    if not tun:
+        return "default"
+    t = {"xcd": 2, "tile": 0, **tun}
+    return "order{order}_bb{bucket_bits}_xcd{xcd}".format(**t) + ("_t16" if t["tile"] else "")
 
 
 def roofline(t, k, path, precision, workload, tun=None, slabs_per_problem=1.0, ns=1,
@@ -483,7 +485,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch choices instead of letkf_gpu_autotune")
     ap.add_argument("--tuning", default=None,
-                    help="launch choices order,bucket_bits[,tridiag_reg,overlap,chunk_log2,xcd] set "
+                    help="launch choices order,bucket_bits[,tridiag_reg,overlap,chunk_log2,xcd,tile] set "
                          "instead of autotuning (as a saved autotune result is reused)")
     ap.add_argument("--cycles", type=int, default=10, help="c5: number of analysis cycles")
     ap.add_argument("--no-graph", action="store_true",
@@ -495,9 +497,9 @@ def main():
     tun_fixed = None
     if args.tuning:
         v = [int(x) for x in args.tuning.split(",")]
-        v += [1, 0, 20, 2][len(v) - 2:] if len(v) < 6 else []
+        v += [1, 0, 20, 2, 0][len(v) - 2:] if len(v) < 7 else []
         tun_fixed = dict(order=v[0], bucket_bits=v[1], tridiag_reg=v[2], overlap=v[3],
-                         chunk_log2=v[4], xcd=v[5])
+                         chunk_log2=v[4], xcd=v[5], tile=v[6])
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, args.config, args.path, args.precision,
                       args.problems_per_dispatch, tun_fixed)
@@ -663,6 +665,35 @@ def main():
                                            "finalize": round(rt["finalize_ms"], 3)}}
         out["speedup_vs_fp64_plain"] = round(out["value"] / out["fp64_plain"]["value"], 3)
         log(f"[bench] fp64 plain: {out['fp64_plain']}")
+        if prob.k < 64:
+            # the plain FP64 path with the batched path's transform (the
+            # rational A^(-1/2) instead of the QL eigensolve): the speed-up of
+            # the precision alone
+            os.environ["LETKF_GPU_RAT"] = "2"
+            try:
+                ref = LetkfGpuSolver(prob, precision="fp64", path="plain")
+            finally:
+                del os.environ["LETKF_GPU_RAT"]
+            if not args.no_autotune:
+                ref.autotune(repeats=1)
+            for _ in range(args.warmup):
+                ref.run()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                ref.run()
+            dt = time.perf_counter() - t0
+            rt = ref.timing()
+            ref.final()
+            out["fp64_plain_rational"] = {
+                "value": round(prob.nij * args.steps / dt, 1),
+                "ms_per_step": round(1e3 * dt / args.steps, 3),
+                "phases_ms": {"search": round(rt["search_ms"], 3),
+                              "product": round(rt["product_ms"], 3),
+                              "transform": round(rt["eigen_ms"], 3),
+                              "finalize": round(rt["finalize_ms"], 3)}}
+            out["speedup_vs_fp64_plain_same_transform"] = round(
+                out["value"] / out["fp64_plain_rational"]["value"], 3)
+            log(f"[bench] fp64 plain, rational transform: {out['fp64_plain_rational']}")
     print(json.dumps(out), flush=True)
 
 

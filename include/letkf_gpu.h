@@ -151,8 +151,12 @@ typedef struct letkf_gpu_timing {
  * chunk_log2: at most 2^chunk_log2 (gridpoint, group) problems per chunk
  * (17..20); xcd: how the product kernel's workgroups meet the 8 XCDs (each
  * with its own L2), 0 in launch order, 1..3 each XCD runs of 32, 64 or 128
- * consecutive problems, the runs of the XCDs interleaved (default 2).  None
- * of them changes the analysis. */
+ * consecutive problems, the runs of the XCDs interleaved (default 2); tile:
+ * the batched product's MFMA tiles, 0 32x32 blocks over rows padded to a
+ * multiple of 32, 1 16x16 tiles over rows padded to a multiple of 16
+ * (BF16 and split-BF16 products, k <= 111; 0 elsewhere).  None of them
+ * changes the analysis, except that the tile shape changes the order of
+ * the FP32 accumulation (the products agree to rounding). */
 typedef struct letkf_gpu_tuning {
   int32_t order;
   int32_t bucket_bits;
@@ -163,6 +167,7 @@ typedef struct letkf_gpu_tuning {
   int32_t overlap;
   int32_t chunk_log2;
   int32_t xcd;
+  int32_t tile;
 } letkf_gpu_tuning;
 
 typedef struct letkf_gpu_t letkf_gpu_t;

@@ -25,5 +25,6 @@ int main(void) {
   F(letkf_gpu_tuning, default_ms);
   F(letkf_gpu_tuning, chunk_log2);
   F(letkf_gpu_tuning, xcd);
+  F(letkf_gpu_tuning, tile);
   return 0;
 }

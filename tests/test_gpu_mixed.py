@@ -84,7 +84,8 @@ def test_launch_choices_and_autotune():
     octave) and the XCD placement (launch order, runs of 32, 64 or 128) only
     reorder independent problems: the analysis is bitwise the same under
     each.  The register (one and two problems per wave) and LDS
-    tridiagonalizations agree to rounding.  autotune times them, keeps the fastest and leaves that
+    tridiagonalizations agree to rounding, and the 16x16-tile products with
+    the 32x32 ones to the products' precision.  autotune times them, keeps the fastest and leaves that
     analysis; bad settings are refused."""
     from letkf_amd.solver import LetkfError, LetkfGpuSolver
     p = synth.random_problem(seed=777, k=40, ns=2, nobs=4000, rtps=0.4)
@@ -104,16 +105,36 @@ def test_launch_choices_and_autotune():
             s.run()
             st, mn, sp = s.state_f64()
             assert rel(st, st0) <= 1e-12 and rel(mn, mn0) <= 1e-12, tr
+        s.set_tuning(2, 0, 1, tile=1)
+        assert s.tuning()["tile"] == 1
+        s.run()
+        st, mn, sp = s.state_f64()
+        assert rel(st, st0) <= 1e-5 and rel(mn, mn0) <= 1e-5
         t = s.autotune(repeats=1)
         print("autotune", t, flush=True)
-        assert t["n_tried"] == 13 and t["best_ms"] <= t["default_ms"]
-        assert s.tuning()["order"] == t["order"]
+        assert t["n_tried"] == 14 and t["best_ms"] <= t["default_ms"]
+        assert s.tuning()["order"] == t["order"] and s.tuning()["tile"] == t["tile"]
         st, mn, sp = s.state_f64()
-        assert rel(st, st0) <= 1e-12 and rel(mn, mn0) <= 1e-12
+        # the tile shape changes the order of the FP32 accumulation only
+        tol = 1e-5 if t["tile"] else 1e-12
+        assert rel(st, st0) <= tol and rel(mn, mn0) <= tol
         for bad in ((4, 0, 1), (2, 3, 1), (2, 0, 3), (2, 0, 1, 2), (2, 0, 1, 1, 16),
-                    (2, 0, 1, 1, 21), (2, 0, 1, 1, 20, 4), (2, 0, 1, 1, 20, -1)):
+                    (2, 0, 1, 1, 21), (2, 0, 1, 1, 20, 4), (2, 0, 1, 1, 20, -1),
+                    (2, 0, 1, 1, 20, 2, 2)):
             with pytest.raises(LetkfError):
                 s.set_tuning(*bad)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16x4", "bf16"])
+@pytest.mark.parametrize("k", [15, 20, 40, 50, 100])
+def test_tile16_products(prec, k, monkeypatch):
+    """The 16x16-tile product kernel (launch choice tile = 1, here through
+    LETKF_GPU_TILE: v_mfma_f32_16x16x32_bf16, rows padded to a multiple of
+    16, 1 to 7 row tiles) meets the same bounds against the oracle as the
+    32x32 kernel; k = 100 runs the large-ensemble transform."""
+    monkeypatch.setenv("LETKF_GPU_TILE", "1")
+    p = synth.random_problem(seed=500 + k, k=k, ns=2, nobs=4000, rtps=0.4)
+    _mixed(p, prec)
 
 
 @pytest.mark.parametrize("prec,path", [("bf16x3", "batched"), ("fp64", "plain")])

@@ -86,6 +86,21 @@ def test_vector_and_full_eigen_paths(case, vec, monkeypatch):
     _check(p)
 
 
+@pytest.mark.parametrize("case", ["k20_ns3", "k50_ns1", "k31_ns8", "vgauss"])
+def test_plain_rational(case, monkeypatch):
+    """LETKF_GPU_RAT=2: the plain FP64 path applies A^(-1/2) by the rational
+    approximation (nodes to 1e-12 relative) instead of the QL eigensolve, the
+    batched path's transform with FP64 products; it meets the same 1e-10
+    bar."""
+    monkeypatch.setenv("LETKF_GPU_RAT", "2")
+    if case == "vgauss":
+        p = synth.random_problem(seed=62, k=40, ns=4, nobs=2500, loc_cls="LOC_VGAUSS", rtps=0.4)
+    else:
+        k, ns = {"k20_ns3": (20, 3), "k50_ns1": (50, 1), "k31_ns8": (31, 8)}[case]
+        p = synth.random_problem(seed=70 + k + ns, k=k, ns=ns, nobs=2500, rtps=0.5)
+    _check(p)
+
+
 def test_k70_explicit_transform():
     """k = 70 is past the tridiagonal-QL pipeline (k < 64): the plain path
     forms the transform matrix explicitly and applies it in FP64."""

@@ -1,6 +1,8 @@
 """In-bounds probe of the MFMA lane layouts the kernels assume (gfx950):
-32x32x2 f32 and 32x32x16 bf16 (operands and 16-register accumulator) and
-16x16x4 f64 (accumulator register r of lane l holds row 4r + l/16)."""
+32x32x2 f32 and 32x32x16 bf16 (operands and 16-register accumulator),
+16x16x32 bf16 (lane l holds k = 8(l/16)..+7 of row l%16; accumulator
+register r of lane l holds row 4(l/16) + r, column l%16) and 16x16x4 f64
+(accumulator register r of lane l holds row 4r + l/16)."""
 import ctypes as C
 import os
 import subprocess
@@ -48,6 +50,15 @@ def test_bf16_32x32x16(probe):
     B = g.integers(-4, 5, (16, 32)).astype(np.float32)
     D = np.zeros((32, 32), np.float32)
     _run(probe, 1, A, B, D)
+    assert np.array_equal(D, A @ B)
+
+
+def test_bf16_16x16x32(probe):
+    g = np.random.default_rng(3)
+    A = g.integers(-4, 5, (16, 32)).astype(np.float32)
+    B = g.integers(-4, 5, (32, 16)).astype(np.float32)
+    D = np.zeros((16, 16), np.float32)
+    _run(probe, 3, A, B, D)
     assert np.array_equal(D, A @ B)
 
 

@@ -11,7 +11,8 @@
 // ahead and columns one step ahead of their MFMAs.  Both MFMA operands are
 // sqrt(rloc_j/err_j) y_j, so each value is rounded or split once; row k of
 // the padded columns carries d_j, so column k of the result is Yb^T R^-1 d.
-// Only the upper-triangle 32x32 blocks are computed.
+// Only the upper-triangle 32x32 blocks are computed (or, as a launch
+// choice, 16x16 tiles: k_product_t16 below).
 //
 //   PREC_FP32    v_mfma_f32_32x32x2_f32   (8 MFMA per block per 16 obs)
 //   PREC_BF16    v_mfma_f32_32x32x16_bf16 (1 per block per 16 obs)
@@ -285,6 +286,205 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same products on 16x16 tiles (v_mfma_f32_16x16x32_bf16): the operand
+// rows are padded to 16 NT >= k + 1 rather than to a multiple of 32, and
+// the upper triangle is NT (NT + 1) / 2 tiles of 16x16 (k = 40: 6 tiles,
+// 1,536 products per observation, against 3 blocks of 32x32, 3,072; k = 50:
+// 2,560 against 3,072).  One step is 32 observations: lane l holds
+// observations 8 (l / 16) + e (e = 0..7) of row 16 t + l % 16 of every row
+// tile t, so each lane reads 8 list entries per 32 observations (half the
+// 32x32 kernel's).  The columns of a step are loaded into one register set
+// right after the previous step's were converted to BF16, and are in flight
+// during that step's MFMAs; lists are loaded two steps ahead.  Split-BF16
+// precisions and compensated accumulation as above (PREC_FP32 keeps the
+// 32x32 kernel).
+// ---------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int NT>
+struct T16Frag {
+  float y[NT][8];   // y_j[16 t + lane%16] of the lane's 8 obs, as loaded
+  float w[8];       // their sqrt(rloc_j/err_j)
+};
+
+__device__ __forceinline__ void t16_list(ListRegs& Lr, __amdgpu_buffer_rsrc_t ir, int j0,
+                                         int lane) {
+  const uint32_t vo = (uint32_t)(j0 + 8 * (lane >> 4)) * 4u;
+  const u32x4 a0 = __builtin_amdgcn_raw_buffer_load_b128(ir, vo, 0, 0);
+  const u32x4 a1 = __builtin_amdgcn_raw_buffer_load_b128(ir, vo, 16, 0);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    Lr.sidx[e] = a0[e];
+    Lr.sidx[4 + e] = a1[e];
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void t16_cols(T16Frag<NT>& F, __amdgpu_buffer_rsrc_t hxr,
+                                         __amdgpu_buffer_rsrc_t wr, const ListRegs& Lr, int j0,
+                                         int lane) {
+  const uint32_t vo = (uint32_t)(j0 + 8 * (lane >> 4)) * 4u;
+  const u32x4 b0 = __builtin_amdgcn_raw_buffer_load_b128(wr, vo, 0, 0);
+  const u32x4 b1 = __builtin_amdgcn_raw_buffer_load_b128(wr, vo, 16, 0);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    F.w[e] = __uint_as_float(b0[e]);
+    F.w[4 + e] = __uint_as_float(b1[e]);
+  }
+  const uint32_t i4 = 4u * (uint32_t)(lane & 15);
+#pragma unroll
+  for (int e = 0; e < 8; ++e)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      F.y[t][e] = __uint_as_float(
+          __builtin_amdgcn_raw_buffer_load_b32(hxr, Lr.sidx[e] + i4, 64 * t, 0));
+}
+
+// weighted operands of a step, split (or rounded) to BF16
+template <int NT, int PREC>
+__device__ __forceinline__ void t16_convert(const T16Frag<NT>& F, bf16x8* hi, bf16x8* lo) {
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    float c[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) c[e] = F.y[t][e] * F.w[e];
+    if (PREC == LETKF_PREC_BF16) hi[t] = to_bf16(c);
+    else split_bf16(c, hi[t], lo[t]);
+  }
+}
+
+// acc[tile (rt, ct), ct >= rt] += products of one step; each term sweeps all
+// tiles, so consecutive MFMAs write different accumulators
+template <int NT, int PREC>
+__device__ __forceinline__ void t16_mfma(f32x4* acc, const bf16x8* hi, const bf16x8* lo) {
+  if (PREC == LETKF_PREC_BF16X4) {
+    int b = 0;
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+      for (int ct = rt; ct < NT; ++ct, ++b)
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[rt], lo[ct], acc[b], 0, 0, 0);
+  }
+  if (PREC != LETKF_PREC_BF16) {
+    int b = 0;
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+      for (int ct = rt; ct < NT; ++ct, ++b)
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[rt], hi[ct], acc[b], 0, 0, 0);
+    b = 0;
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+      for (int ct = rt; ct < NT; ++ct, ++b)
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[rt], lo[ct], acc[b], 0, 0, 0);
+  }
+  int b = 0;
+#pragma unroll
+  for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+    for (int ct = rt; ct < NT; ++ct, ++b)
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[rt], hi[ct], acc[b], 0, 0, 0);
+}
+
+// compensated accumulation on 16x16 tiles (4 values per lane per tile)
+template <int NTILE>
+__device__ __forceinline__ void t16_fold(f32x4* acc, f32x4* S, float* carry, int lane) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int b = 0; b < NTILE; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float a = acc[b][r], x = S[b][r];
+      const float t = x + a;
+      const float bb = t - x;
+      const float e = (x - (t - bb)) + (a - bb);
+      S[b][r] = t;
+      carry[(4 * b + r) * 64 + lane] += e;
+      acc[b][r] = 0.0f;
+    }
+}
+
+template <int NT, int PREC>
+__global__ __launch_bounds__(64) void k_product_t16(ObsDev O, int k, ChunkDev C,
+                                                    const int32_t* __restrict__ perm,
+                                                    double* __restrict__ B,
+                                                    double* __restrict__ bvec, int fold_mask) {
+  constexpr int NTILE = NT * (NT + 1) / 2;
+  const int64_t pair = perm[xcd_order(C.xcd, blockIdx.x, gridDim.x)];
+  if (!C.solve[pair]) return;
+  const int m = (int)C.cnt[pair];
+  const int64_t off = C.off[pair];
+  const int lane = threadIdx.x;
+  const __amdgpu_buffer_rsrc_t hxr = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)O.hxp, (short)0, (int)(O.n * O.kp * 4), 0x00020000);
+  const int lbytes = ((m + kListAlign - 1) & ~(kListAlign - 1)) * 4;
+  const __amdgpu_buffer_rsrc_t ir =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(C.list_idx + off), (short)0, lbytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t wr =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(C.list_w + off), (short)0, lbytes, 0x00020000);
+
+  constexpr bool COMP = PREC == LETKF_PREC_BF16X4;
+  f32x4 acc[NTILE];
+#pragma unroll
+  for (int b = 0; b < NTILE; ++b) acc[b] = (f32x4){};
+  f32x4 S[COMP ? NTILE : 1];
+  __shared__ float carry[COMP ? NTILE * 4 * 64 : 1];
+  if (COMP) {
+#pragma unroll
+    for (int b = 0; b < NTILE; ++b) S[b] = (f32x4){};
+#pragma unroll
+    for (int i = 0; i < NTILE * 4; ++i) carry[i * 64 + lane] = 0.0f;
+  }
+
+  // f: the columns of the step in flight; la, lb: lists two and one steps
+  // ahead (steps past m load zeros and add nothing)
+  T16Frag<NT> f;
+  ListRegs la, lb;
+  bf16x8 hi[NT], lo[PREC == LETKF_PREC_BF16 ? 1 : NT];
+  t16_list(la, ir, 0, lane);
+  t16_list(lb, ir, 32, lane);
+  t16_cols<NT>(f, hxr, wr, la, 0, lane);
+  t16_list(la, ir, 64, lane);
+  for (int j0 = 0; j0 < m; j0 += 32) {
+    __builtin_amdgcn_sched_barrier(0);
+    t16_convert<NT, PREC>(f, hi, lo);
+    __builtin_amdgcn_sched_barrier(0);
+    t16_cols<NT>(f, hxr, wr, lb, j0 + 32, lane);
+    lb = la;
+    t16_list(la, ir, j0 + 96, lane);
+    __builtin_amdgcn_sched_barrier(0);
+    t16_mfma<NT, PREC>(acc, hi, lo);
+    if (COMP && ((j0 + 32) & fold_mask) == 0) t16_fold<NTILE>(acc, S, carry, lane);
+  }
+
+  // 16x16 f32 accumulator: register r of lane l holds row 4 (l / 16) + r,
+  // column l % 16
+  double* Bp = B + pair * (int64_t)k * k;
+  double* bp = bvec + pair * (int64_t)k;
+  const int g = lane >> 4, i = lane & 15;
+  int b = 0;
+#pragma unroll
+  for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+    for (int ct = rt; ct < NT; ++ct, ++b) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * rt + 4 * g + r;
+        const int col = 16 * ct + i;
+        double v = (double)acc[b][r];
+        if (COMP) v = (double)S[b][r] + (v + (double)carry[(4 * b + r) * 64 + lane]);
+        if (row < k && col < k) {
+          Bp[row + (int64_t)k * col] = v;
+          Bp[col + (int64_t)k * row] = v;
+        } else if (col == k && row < k) {
+          bp[row] = v;
+        }
+      }
+    }
+}
+
 template <int KP>
 static void launch_kp(hipStream_t st, int prec, unsigned nb, const ObsDev& O, int k,
                       const ChunkDev& C, const int32_t* perm, double* B, double* bvec,
@@ -299,15 +499,43 @@ static void launch_kp(hipStream_t st, int prec, unsigned nb, const ObsDev& O, in
     k_product_mx<KP, LETKF_PREC_BF16X3><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec, fm);
 }
 
+template <int NT>
+static void launch_nt(hipStream_t st, int prec, unsigned nb, const ObsDev& O, int k,
+                      const ChunkDev& C, const int32_t* perm, double* B, double* bvec,
+                      int fm) {
+  if (prec == LETKF_PREC_BF16)
+    k_product_t16<NT, LETKF_PREC_BF16><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec, fm);
+  else if (prec == LETKF_PREC_BF16X4)
+    k_product_t16<NT, LETKF_PREC_BF16X4><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec, fm);
+  else
+    k_product_t16<NT, LETKF_PREC_BF16X3><<<nb, 64, 0, st>>>(O, k, C, perm, B, bvec, fm);
+}
+
+bool product_t16_supported(int prec, int k) {
+  return prec != LETKF_PREC_FP32 && (k + 1 + 15) / 16 <= 7;
+}
+
 void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev& S,
                        const ChunkDev& C, const int32_t* perm, double* B, double* bvec,
-                       int fold_log2) {
+                       int fold_log2, int tile) {
   const int64_t np = C.ngp * C.G;
   if (np <= 0) return;
   const int k = (int)S.k;
   const int kp = ((k + 1 + 31) / 32) * 32;
   const unsigned nb = (unsigned)np;
-  const int fm = (1 << (fold_log2 < 5 ? 5 : fold_log2)) - 1;   // the loop steps by 32
+  const int fm = (1 << (fold_log2 < 5 ? 5 : fold_log2)) - 1;   // the loops step by 32
+  if (tile == 1 && product_t16_supported(prec, k)) {
+    switch ((k + 1 + 15) / 16) {
+      case 1: launch_nt<1>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
+      case 2: launch_nt<2>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
+      case 3: launch_nt<3>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
+      case 4: launch_nt<4>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
+      case 5: launch_nt<5>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
+      case 6: launch_nt<6>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
+      default: launch_nt<7>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
+    }
+    return;
+  }
   switch (kp) {
     case 32: launch_kp<32>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;
     case 64: launch_kp<64>(st, prec, nb, O, k, C, perm, B, bvec, fm); break;

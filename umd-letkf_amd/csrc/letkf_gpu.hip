@@ -143,6 +143,7 @@ struct letkf_gpu_t {
   size_t live_bytes = 0;
   size_t mem_limit = 0;   // 0 = none
   int tune_xcd = 2;       // product kernels: runs of 64 problems per XCD
+  int tune_tile = 0;      // batched product: 0 32x32 MFMA blocks, 1 16x16 tiles
   int fold_log2 = 7;      // compensated products: partial sums folded every 2^n obs
   int tune_bucket = 2;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
   int tune_bbits = 0;     // bucket edges: powers of two split in 2^bbits parts
@@ -151,7 +152,8 @@ struct letkf_gpu_t {
   int tune_overlap = 0;   // search of the next chunk concurrent with the solve
   int tune_chunk_log2 = 20;   // at most 2^n problems per chunk (17..20)
   int tune_vec = 1;       // vector path when every group has few slabs
-  int tune_rat = 1;       // batched vector path: rational A^(-1/2), no eigensolve
+  int tune_rat = 1;       // vector path: rational A^(-1/2), no eigensolve (1 batched
+                          // path only, 2 the plain FP64 path too, 0 off)
   int tune_mat = 1;       // batched, 64 <= k <= 128: explicit T applied on MFMA
   bool use_mat = false;
   int alloc_mat = -1;
@@ -297,6 +299,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   if (const char* e = getenv("LETKF_GPU_MEM_LIMIT")) h->mem_limit = strtoull(e, nullptr, 10);
   if (const char* e = getenv("LETKF_GPU_XCD")) h->tune_xcd = std::min(3, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_FOLD")) h->fold_log2 = std::min(10, std::max(5, atoi(e)));
+  if (const char* e = getenv("LETKF_GPU_TILE")) h->tune_tile = atoi(e) != 0;
   if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = std::min(3, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_BBITS")) h->tune_bbits = std::min(2, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_TRIREG")) h->tune_trireg = std::min(2, std::max(0, atoi(e)));
@@ -845,7 +848,7 @@ static int ensure_outputs(letkf_gpu_t* h) {
   HIPCHK(h, dalloc(h, &h->d_lgloc, (size_t)(G * nij)));
   HIPCHK(h, dalloc(h, &h->d_lghash, (size_t)(G * nij)));
   // chunk sizing: bound the per-pair work arrays to about 32 GB
-  h->use_rat = nv > 0 && h->batched() && h->tune_rat && k >= 2;
+  h->use_rat = nv > 0 && (h->batched() ? h->tune_rat != 0 : h->tune_rat == 2) && k >= 2;
   h->cap_rot = h->cap_it = 0;
   if (h->eigK && !h->use_rat) eigen_caps((int)k, &h->cap_rot, &h->cap_it);
   const int64_t per_pair = 8 * k * k + 8 * k + 3 * 8 * (int64_t)(use_mat ? 128 : h->eigK) +
@@ -1143,7 +1146,7 @@ static int run_eager(letkf_gpu_t* h) {
                                                 h->d_bval, h->d_perm, (int)np, 0, 32, st));
       RUNCHK(hipEventRecord(ce[3], st));
       launch_product_mx(st, h->dims.precision, O, S, C, h->d_perm, h->d_B, h->d_b,
-                        h->fold_log2);
+                        h->fold_log2, h->tune_tile);
     } else {
       RUNCHK(hipEventRecord(ce[3], st));
       launch_product_f64(st, O, S, C, h->d_B, h->d_b);
@@ -1225,7 +1228,7 @@ static hipError_t enqueue_analysis(letkf_gpu_t* h) {
                                              h->d_perm, (int)np, 0, 32, st);
       if (e != hipSuccess) break;
       launch_product_mx(st, h->dims.precision, W.O, W.S, C, h->d_perm, h->d_B, h->d_b,
-                        h->fold_log2);
+                        h->fold_log2, h->tune_tile);
     } else {
       launch_product_f64(st, W.O, W.S, C, h->d_B, h->d_b);
     }
@@ -1570,7 +1573,7 @@ int letkf_gpu_get_timing(const letkf_gpu_t* h, letkf_gpu_timing* t) {
 int letkf_gpu_get_tuning(const letkf_gpu_t* h, letkf_gpu_tuning* out) {
   if (!h || !out) return LETKF_EINVAL;
   *out = letkf_gpu_tuning{h->tune_bucket, h->tune_bbits, h->tune_trireg, 0, 0.0, 0.0,
-                          h->tune_overlap, h->tune_chunk_log2, h->tune_xcd};
+                          h->tune_overlap, h->tune_chunk_log2, h->tune_xcd, h->tune_tile};
   return LETKF_OK;
 }
 
@@ -1588,6 +1591,8 @@ int letkf_gpu_set_tuning(letkf_gpu_t* h, const letkf_gpu_tuning* t) {
     return fail(h, LETKF_EINVAL, "tuning chunk_log2 %d outside 17..20", t->chunk_log2);
   if (t->xcd < 0 || t->xcd > 3)
     return fail(h, LETKF_EINVAL, "tuning xcd %d outside 0..3", t->xcd);
+  if (t->tile != 0 && t->tile != 1)
+    return fail(h, LETKF_EINVAL, "tuning tile %d not 0 or 1", t->tile);
   ++h->gen;
   h->tune_bucket = t->order;
   h->tune_bbits = t->bucket_bits;
@@ -1595,6 +1600,7 @@ int letkf_gpu_set_tuning(letkf_gpu_t* h, const letkf_gpu_tuning* t) {
   h->tune_overlap = t->overlap;
   h->tune_chunk_log2 = t->chunk_log2;
   h->tune_xcd = t->xcd;
+  h->tune_tile = t->tile;
   return LETKF_OK;
 }
 
@@ -1649,8 +1655,14 @@ int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out) {
         if ((rc = consider(t))) return rc;
       }
   }
-  // the product kernel's XCD placement (batched path: that kernel reads the
-  // observation columns neighbouring problems share)
+  // the product kernel's tile shape (32x32 blocks or 16x16 tiles, where the
+  // 16x16 kernel exists), then its XCD placement (batched path: that kernel
+  // reads the observation columns neighbouring problems share)
+  if (h->batched() && product_t16_supported(h->dims.precision, (int)h->k)) {
+    letkf_gpu_tuning t = best;
+    t.tile = 1 - t.tile;
+    if ((rc = consider(t))) return rc;
+  }
   if (h->batched()) {
     const letkf_gpu_tuning base = best;
     for (int x = 0; x <= 3; ++x) {

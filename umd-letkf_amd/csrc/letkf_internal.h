@@ -194,9 +194,12 @@ hipError_t launch_eigen_apply_f64(hipStream_t st, const LocDev& L, const StateDe
                                   const VarsDev& V, const ChunkDev& C, const DiagDev& D,
                                   const double* B, const double* bvec, const int* only_flagged,
                                   double* ana);
+// tile 0: 32x32 MFMA blocks; tile 1: 16x16 tiles where product_t16_supported
 void launch_product_mx(hipStream_t st, int prec, const ObsDev& O, const StateDev& S,
                        const ChunkDev& C, const int32_t* perm, double* B, double* bvec,
-                       int fold_log2);
+                       int fold_log2, int tile);
+// a 16x16-tile product kernel exists (BF16 and split-BF16, k <= 111)
+bool product_t16_supported(int prec, int k);
 void launch_bucket_keys(hipStream_t st, int mode, int bbits, int64_t np, int G, int64_t i0,
                         const int64_t* cnt, const uint8_t* solve, const float* lat,
                         const float* lon, uint32_t* keys, int32_t* vals);

@@ -83,7 +83,7 @@ MODULE letkf_gpu_mod
   TYPE, BIND(C), PUBLIC :: letkf_gpu_tuning
      INTEGER(c_int32_t) :: order, bucket_bits, tridiag_reg, n_tried
      REAL(c_double) :: best_ms, default_ms
-     INTEGER(c_int32_t) :: overlap, chunk_log2, xcd
+     INTEGER(c_int32_t) :: overlap, chunk_log2, xcd, tile
   END TYPE letkf_gpu_tuning
 
   PUBLIC :: letkf_gpu_version, letkf_gpu_create, letkf_gpu_destroy, letkf_gpu_last_error, &

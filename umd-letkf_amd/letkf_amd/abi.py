@@ -87,7 +87,8 @@ class Tuning(C.Structure):
     _fields_ = [("order", C.c_int32), ("bucket_bits", C.c_int32),
                 ("tridiag_reg", C.c_int32), ("n_tried", C.c_int32),
                 ("best_ms", C.c_double), ("default_ms", C.c_double),
-                ("overlap", C.c_int32), ("chunk_log2", C.c_int32), ("xcd", C.c_int32)]
+                ("overlap", C.c_int32), ("chunk_log2", C.c_int32), ("xcd", C.c_int32),
+                ("tile", C.c_int32)]
 
 
 EXPORTED = [

@@ -217,8 +217,9 @@ class LetkfGpuSolver:
         return {f: getattr(t, f) for f, _ in abi.Tuning._fields_}
 
     def set_tuning(self, order: int, bucket_bits: int = 0, tridiag_reg: int = 1,
-                   overlap: int = 1, chunk_log2: int = 20, xcd: int = 2, **_):
-        t = abi.Tuning(order, bucket_bits, tridiag_reg, 0, 0.0, 0.0, overlap, chunk_log2, xcd)
+                   overlap: int = 1, chunk_log2: int = 20, xcd: int = 2, tile: int = 0, **_):
+        t = abi.Tuning(order, bucket_bits, tridiag_reg, 0, 0.0, 0.0, overlap, chunk_log2, xcd,
+                       tile)
         self._chk(self.lib.letkf_gpu_set_tuning(self.h, C.byref(t)))
 
     def final(self):
