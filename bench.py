@@ -63,8 +63,15 @@ FETCH_SCALE = 2.0
 WRITE_SCALE = 1.0
 
 
-def product_kernel(path, precision):
-    return "k_product_f64" if path == "plain" else "k_product_mx"
+def product_kernel(path, precision, tun=None, k=None):
+    """The product kernel a run launches: the plain path's FP64 kernel, or the
+    batched path's 16x16-tile kernel when the launch choice asks for it and
+    it exists (BF16 and split-BF16 products, k <= 111), else the 32x32 one."""
+    if path == "plain":
+        return "k_product_f64"
+    t16 = (tun or {}).get("tile", 0) and precision != "fp32" and (
+        k is None or (k + 1 + 15) // 16 <= 7)
+    return "k_product_t16" if t16 else "k_product_mx"
 
 
 def tuning_key(tun):
@@ -89,7 +96,7 @@ def roofline(t, k, path, precision, workload, tun=None, slabs_per_problem=1.0, n
     nl = max(int(t["n_launch"]), 1)
     peak = PEAK["fp64"] if path == "plain" else PEAK[precision]
     ach = flop / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
-    kern = product_kernel(path, precision)
+    kern = product_kernel(path, precision, tun, k)
     gather = t["n_pairs"] * (4.0 * k + 8.0)          # hx column + (index, weight)
     roof = dict(kernel=kern, bound="mfma", achieved=round(ach, 3), peak=peak, unit="TFLOP/s",
                 frac=round(ach / peak, 5), traffic=None,
@@ -134,7 +141,7 @@ def summarize_pmc(fetch_csv, write_csv, workload, path, precision, problems, tun
     """Per-dispatch HBM traffic of the product kernel from two rocprofv3
     --pmc runs, merged into profiles/traffic.json."""
     import csv
-    kern = product_kernel(path, precision)
+    kern = product_kernel(path, precision, tun)
 
     def per_dispatch(fn, counter):
         vals = [float(r["Counter_Value"]) for r in csv.DictReader(open(fn))
