@@ -49,3 +49,23 @@ def test_pair_matches_one_problem_and_oracle(precision, path, k, ns):
         e["state"] = rel(st, orc["state"])
     print("pair vs oracle", e, flush=True)
     assert max(e.values()) <= tol, e
+
+
+@pytest.mark.parametrize("ns", [1, 2])
+@pytest.mark.parametrize("k", [32, 49, 50])
+def test_rational_solve_fused(k, ns, monkeypatch):
+    """On the batched path the rational solve runs inside the two-problem
+    tridiagonalization (default) or as its own kernel (LETKF_GPU_FUSE=0):
+    the same arithmetic on the same values, so the analyses agree to
+    rounding (1e-12 relative), and the fused one meets the oracle (1e-5)."""
+    p = synth.random_problem(seed=950 + k + ns, nx=47, ny=23, k=k, ns=ns, nobs=2500, rtps=0.3)
+    print("fused rational solve k", k, "ns", ns, flush=True)
+    fused = _run(p, "bf16x4", "batched", 2)
+    monkeypatch.setenv("LETKF_GPU_FUSE", "0")
+    apart = _run(p, "bf16x4", "batched", 2)
+    d = {f: rel(a, b) for f, a, b in zip(("state", "mean", "sprd"), fused[:3], apart[:3])}
+    print("fused vs separate", d, flush=True)
+    assert max(d.values()) <= 1e-12, d
+    orc = oracle_run(p)
+    e = dict(mean=rel(fused[1], orc["mean"]), sprd=rel(fused[2], orc["sprd"]))
+    assert max(e.values()) <= 1e-5, e

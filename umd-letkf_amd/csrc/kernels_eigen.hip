@@ -730,25 +730,28 @@ static void tql_launch(hipStream_t st, unsigned nl, int k, int64_t np, const Chu
   k_tql<KQ, K><<<nl, 64, 0, st>>>(k, np, C, dd, ee, rot, cap_rot, itr, cap_it, nit, status);
 }
 
+// returns 2 if the rational solve ran in the same kernel (R given), else 1
 template <int K, int NV>
-static void tridiag_launch(hipStream_t st, unsigned nb, size_t lds, int k, const LocDev& L,
-                           const StateDev& S, const VarsDev& V, const ChunkDev& C,
-                           const DiagDev& D, double* B, const double* bvec, double* tau,
-                           double* dd, double* ee, double* U, int* nvec, int* status,
-                           int reg, const int32_t* perm) {
+static int tridiag_launch(hipStream_t st, unsigned nb, size_t lds, int k, const LocDev& L,
+                          const StateDev& S, const VarsDev& V, const ChunkDev& C,
+                          const DiagDev& D, double* B, const double* bvec, double* tau,
+                          double* dd, double* ee, double* U, int* nvec, int* status,
+                          int reg, const int32_t* perm, const RatTab& R) {
   // k >= 32 (K = 64), rows padded to a multiple of 8 (to 50 for k = 49 and
   // 50): reg 2 the two-problem register kernel (k <= 50), reg 1 (or 2 past
   // k = 50) the one-problem register kernel, reg 0 the LDS kernel
   const int kq = (k == 49 || k == 50) ? 50 : (k + 7) / 8 * 8;
-  if (K == 64 && reg == 2 &&
-      launch_tridiag_pair(st, (int64_t)nb, perm, kq, NV, k, K, L, S, V, C, D, B, bvec, tau, dd,
-                          ee, U, nvec, status))
-    return;
+  if (K == 64 && reg == 2) {
+    const int rc = launch_tridiag_pair(st, (int64_t)nb, perm, kq, NV, k, K, L, S, V, C, D, B,
+                                       bvec, tau, dd, ee, U, nvec, status, R.tab, R.n, R.nb);
+    if (rc) return rc;
+  }
   if (K == 64 && reg != 0 &&
       launch_tridiag_reg(st, nb, kq, NV, k, K, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec,
                          status))
-    return;
+    return 1;
   k_tridiag<K, NV><<<nb, K, lds, st>>>(k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status);
+  return 1;
 }
 
 template <int KQ>
@@ -764,22 +767,29 @@ static void pipeline(hipStream_t st, const LocDev& L, const StateDev& S, const V
                      double* tau, double* dd, double* ee, double2* rot, int cap_rot,
                      int4* itr, int cap_it, int* nit, int* status, double* Z, int NV,
                      double* U, int* nvec, const RatTab& R, double* ana, int reg,
-                     const int32_t* perm) {
+                     const int32_t* perm, bool fuse) {
   const int64_t np = C.ngp * C.G;
   const int k = (int)S.k;
   const unsigned nb = (unsigned)np;
   const unsigned nl = (unsigned)((np + 63) / 64);
   const size_t lds = sizeof(double) * (size_t)k * (k + 1);
+  // the rational solve may run inside the two-problem tridiagonalization
+  const RatTab RF = (NV > 0 && fuse) ? R : RatTab{};
+  int tri = 1;
   switch (NV) {
-    case 0: tridiag_launch<K, 0>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm); break;
-    case 2: tridiag_launch<K, 2>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm); break;
-    case 3: tridiag_launch<K, 3>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm); break;
-    case 5: tridiag_launch<K, 5>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm); break;
-    default: tridiag_launch<K, 9>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm); break;
+    case 0: tri = tridiag_launch<K, 0>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm, RF); break;
+    case 2: tri = tridiag_launch<K, 2>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm, RF); break;
+    case 3: tri = tridiag_launch<K, 3>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm, RF); break;
+    case 5: tri = tridiag_launch<K, 5>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm, RF); break;
+    default: tri = tridiag_launch<K, 9>(st, nb, lds, k, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, reg, perm, RF); break;
   }
   if (NV > 0 && R.nb > 0) {
     // rational T^(-1/2): no QL, no eigenvectors
     const unsigned nr = (unsigned)((np + 1) / 2);
+    if (tri == 2) {   // solved inside the tridiagonalization
+      k_qapply<K><<<nb, K, 0, st>>>(L, S, V, C, D, NV, B, tau, nvec, status, U, ana);
+      return;
+    }
     // rows of 50 for k = 49 and 50 (as the register tridiagonalization)
     switch (k == 49 || k == 50 ? 50 : (k + 7) / 8) {
       case 1: ratsolve_launch<8>(st, nr, k, K, NV, np, C, dd, ee, nvec, status, R, U); break;
@@ -819,14 +829,14 @@ void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
                            double* ee, double2* rot, int cap_rot, int4* itr,
                            int cap_it, int* nit, int* status, double* Z, int NV, double* U,
                            int* nvec, const RatTab& R, double* ana, int trireg,
-                           const int32_t* perm) {
+                           const int32_t* perm, bool fuse) {
   if (C.ngp * C.G <= 0) return;
   if (eigen_pipeline_K((int)S.k) == 32)
     pipeline<32>(st, L, S, V, C, D, B, bvec, tau, dd, ee, rot, cap_rot, itr, cap_it, nit,
-                 status, Z, NV, U, nvec, R, ana, trireg, perm);
+                 status, Z, NV, U, nvec, R, ana, trireg, perm, fuse);
   else
     pipeline<64>(st, L, S, V, C, D, B, bvec, tau, dd, ee, rot, cap_rot, itr, cap_it, nit,
-                 status, Z, NV, U, nvec, R, ana, trireg, perm);
+                 status, Z, NV, U, nvec, R, ana, trireg, perm, fuse);
 }
 
 }  // namespace lk

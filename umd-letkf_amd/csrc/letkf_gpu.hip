@@ -152,6 +152,7 @@ struct letkf_gpu_t {
   int tune_overlap = 0;   // search of the next chunk concurrent with the solve
   int tune_chunk_log2 = 20;   // at most 2^n problems per chunk (17..20)
   int tune_vec = 1;       // vector path when every group has few slabs
+  int tune_fuse = 1;      // rational solve inside the two-problem tridiagonalization
   int tune_rat = 1;       // vector path: rational A^(-1/2), no eigensolve (1 batched
                           // path only, 2 the plain FP64 path too, 0 off)
   int tune_mat = 1;       // batched, 64 <= k <= 128: explicit T applied on MFMA
@@ -300,6 +301,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   if (const char* e = getenv("LETKF_GPU_XCD")) h->tune_xcd = std::min(3, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_FOLD")) h->fold_log2 = std::min(10, std::max(5, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_TILE")) h->tune_tile = atoi(e) != 0;
+  if (const char* e = getenv("LETKF_GPU_FUSE")) h->tune_fuse = atoi(e) != 0;
   if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = std::min(3, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_BBITS")) h->tune_bbits = std::min(2, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_TRIREG")) h->tune_trireg = std::min(2, std::max(0, atoi(e)));
@@ -992,7 +994,8 @@ static hipError_t enqueue_solve(letkf_gpu_t* h, hipStream_t st, const RunViews& 
     launch_eigen_pipeline(st, h->loc, W.S, W.V, C, W.D, h->d_B, h->d_b, h->d_tau, h->d_dd,
                           h->d_ee, h->d_rot, h->cap_rot, h->d_itr, h->cap_it, h->d_nit,
                           h->d_status, h->d_Z, h->vec_nv, h->d_U, h->d_nvec, R, h->d_ana,
-                          h->tune_trireg, h->batched() ? h->d_perm : nullptr);
+                          h->tune_trireg, h->batched() ? h->d_perm : nullptr,
+                          h->tune_fuse != 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // problems the QL could not finish go to the Jacobi solver

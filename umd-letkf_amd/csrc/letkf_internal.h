@@ -212,12 +212,14 @@ bool launch_tridiag_reg(hipStream_t st, unsigned nb, int KQ, int NV, int k, int 
                         int* status);
 // the same reduction for two problems per wave (rows of KQ <= 50, at most
 // 3 carried vectors), the problems taken in the order perm (nullptr:
-// 0..np-1); false if no kernel exists for (KQ, NV)
-bool launch_tridiag_pair(hipStream_t st, int64_t np, const int32_t* perm, int KQ, int NV, int k,
-                         int K, const LocDev& L, const StateDev& S, const VarsDev& V,
-                         const ChunkDev& C, const DiagDev& D, double* B, const double* bvec,
-                         double* tau, double* dd, double* ee, double* U, int* nvec,
-                         int* status);
+// 0..np-1).  With rnb > 0 (rational node tables) and carried vectors the
+// rational solve runs in the same kernel.  Returns 0 if no kernel exists
+// for (KQ, NV), 1 for the reduction alone, 2 with the rational solve.
+int launch_tridiag_pair(hipStream_t st, int64_t np, const int32_t* perm, int KQ, int NV, int k,
+                        int K, const LocDev& L, const StateDev& S, const VarsDev& V,
+                        const ChunkDev& C, const DiagDev& D, double* B, const double* bvec,
+                        double* tau, double* dd, double* ee, double* U, int* nvec,
+                        int* status, const double* rtab, const int* rtabn, int rnb);
 int eigen_pipeline_K(int k);
 void eigen_caps(int k, int* cap_rot, int* cap_it);
 // vectors carried per problem on the vector path (b + slabs of a group,
@@ -241,7 +243,7 @@ void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
                            double* ee, double2* rot, int cap_rot, int4* itr,
                            int cap_it, int* nit, int* status, double* Z, int NV, double* U,
                            int* nvec, const RatTab& R, double* ana, int trireg,
-                           const int32_t* perm);
+                           const int32_t* perm, bool fuse);
 // explicit transform matrix and its MFMA application (64 <= k <= 128)
 bool transform_mat_supported(int k);
 void launch_transform_mat(hipStream_t st, int prec, const LocDev& L, const StateDev& S,

@@ -64,13 +64,125 @@ __device__ __forceinline__ double half_sum(double x, bool upper) {
   return half_bcast(x, 31, upper);
 }
 
+// largest value over this wave's half (x >= 0; whole wave active)
+__device__ __forceinline__ double half_max(double x, bool upper) {
+  x = fmax(x, dpp_z<0x111, 0xf>(x));
+  x = fmax(x, dpp_z<0x112, 0xf>(x));
+  x = fmax(x, dpp_z<0x114, 0xf>(x));
+  x = fmax(x, dpp_z<0x118, 0xf>(x));
+  x = fmax(x, dpp_z<0x142, 0xa>(x));
+  return half_bcast(x, 31, upper);
+}
+
+// 1/x: the hardware estimate and one Newton step (the pivots of the
+// rational solve are >= k - 1 > 0)
+__device__ __forceinline__ double rcp_nr1(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  return y * (2.0 - x * y);
+}
+
+// The rational solve of one problem per half (see k_tridiag_pair, RAT); all
+// lanes of the wave take part (barrier, DPP sums), lanes of an inactive half
+// or of a problem with more slabs than carried vectors store nothing.
 template <int KQ, int NV>
+__device__ __forceinline__ void rat_tail(int k, int K, bool act, int64_t pq, int nsl, int r,
+                                         bool upper, double dg0, double dg1,
+                                         const double (&xv0)[NV], const double (&xv1)[NV],
+                                         double* s_d, double* s_e, double* s_u,
+                                         double* __restrict__ U, const double* __restrict__ rtab,
+                                         const int* __restrict__ rtabn, int rnb) {
+  constexpr int LDH = 66;
+  const int hh = upper ? 1 : 0, r1 = r + 32;
+  double* sdh = s_d + hh * LDH;
+  const double* seh = s_e + hh * LDH;
+  sdh[r] = r < k ? dg0 : 0.0;
+  sdh[r1] = r1 < k ? dg1 : 0.0;
+#pragma unroll
+  for (int qv = 0; qv < NV; ++qv) {
+    s_u[(hh * NV + qv) * LDH + r] = xv0[qv];
+    s_u[(hh * NV + qv) * LDH + r1] = xv1[qv];
+  }
+  __syncthreads();
+  const bool valid = act && nsl <= NV - 1;
+  // Gershgorin bound of the largest eigenvalue of T (SPD)
+  double gm = 0.0;
+  if (r < k) gm = sdh[r] + (r > 0 ? fabs(seh[r - 1]) : 0.0) + fabs(seh[r]);
+  if (r1 < k) gm = fmax(gm, sdh[r1] + fabs(seh[r1 - 1]) + fabs(seh[r1]));
+  gm = half_max(gm, upper);
+  const double m = (double)(k - 1);
+  const double kappa = gm / m;
+  int b = (int)ceil(log2(fmax(kappa, 2.0)));
+  b = b < 1 ? 1 : (b > rnb - 1 ? rnb - 1 : b);
+  const int nn = rtabn[b];
+  const bool node = r < 31;
+  const double sigma = node ? (r < nn ? m * rtab[b * 64 + r] : m) : 0.0;
+  const double coef = (node && r < nn) ? m * rtab[b * 64 + 32 + r] : 0.0;
+  const int nv_h = valid ? nsl : 0;
+  const int nsl_all = max(nv_h, __shfl_xor(nv_h, 32, 64));
+  for (int q = 1; q <= nsl_all; ++q) {
+    const bool actq = valid && q <= nsl;
+    const double* ux = s_u + (hh * NV + (actq ? q : 0)) * LDH;
+    const double* u = node ? ux : s_u + hh * NV * LDH;
+    // LDL^T of T + sigma I with the forward substitution, then back
+    double l[KQ], p[KQ];
+    double rprev = 0.0, zprev = 0.0, eprev = 0.0;
+#pragma unroll
+    for (int i = 0; i < KQ; ++i) {
+      if (i < k) {
+        const double li = eprev * rprev;
+        const double Di = sdh[i] + sigma - li * eprev;
+        const double ri = rcp_nr1(Di);
+        const double zi = u[i] - li * zprev;
+        l[i] = li;
+        p[i] = zi * ri;
+        rprev = ri;
+        zprev = zi;
+        eprev = seh[i];
+      } else {
+        l[i] = 0.0;
+        p[i] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int i = KQ - 2; i >= 0; --i)
+      if (i + 1 < k) p[i] -= l[i + 1] * p[i + 1];
+    // lane 31: wbar . x_b = u_q . T^-1 u_b
+    double wx = 0.0;
+    if (!node) {
+#pragma unroll
+      for (int i = 0; i < KQ; ++i)
+        if (i < k) wx += ux[i] * p[i];
+    }
+    // y_i = sum over the node lanes of coef * x_i, gathered by DPP in the
+    // last lane of each half (the T^-1 b lane, whose coef is 0)
+#pragma unroll
+    for (int i = 0; i < KQ; ++i)
+      if (i < k) p[i] = half_sum_dpp(coef * p[i]);
+    if (actq && !node) {
+      double* uo = U + (pq * NV + q) * K;
+#pragma unroll
+      for (int i = 0; i < KQ; ++i)
+        if (i < k) uo[i] = p[i];
+      uo[K - 1] = wx;
+    }
+  }
+}
+
+// RAT: the rational solve of the batched vector path runs in the same
+// kernel after the reduction (T, Q^T b and Q^T x_b of each half are in
+// registers and go through LDS), in the layout of k_ratsolve: lane j < 31
+// of a half solves (T + sigma_j I) x = Q^T x_b by LDL^T, lane 31 solves
+// T x = Q^T b, and DPP sums over the half form y = sqrt(k-1) T^(-1/2) Q^T x_b
+// and wbar . x_b, written to U for k_qapply (T, Q^T b and the inputs stay
+// on chip: no dd, ee or input U round trip, no separate launch).
+template <int KQ, int NV, bool RAT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_tridiag_pair(int k, int K, int64_t np, const int32_t* __restrict__ perm, LocDev L,
                     StateDev S, VarsDev V, ChunkDev C, DiagDev D, double* __restrict__ B,
                     const double* __restrict__ bvec, double* __restrict__ tau,
                     double* __restrict__ dd, double* __restrict__ ee, double* __restrict__ U,
-                    int* __restrict__ nvec, int* __restrict__ status) {
+                    int* __restrict__ nvec, int* __restrict__ status,
+                    const double* __restrict__ rtab, const int* __restrict__ rtabn, int rnb) {
   // per half: v and w of rows 0..63; the halves' arrays 66 entries apart
   // (a 16-byte read of each half falls in different banks)
   constexpr int LDH = 66;
@@ -81,6 +193,9 @@ void k_tridiag_pair(int k, int K, int64_t np, const int32_t* __restrict__ perm, 
   __shared__ double sv[2 * LDH];        // v, for the symmetric product
   __shared__ double2 svw[2 * LDH];      // (v, w), for the rank-2 update
   __shared__ int sl[2][NV > 0 ? NV : 1];
+  // RAT: T (diagonal, off-diagonal) and the carried vectors of each half
+  __shared__ double s_d[RAT ? 2 * LDH : 1], s_e[RAT ? 2 * LDH : 1];
+  __shared__ double s_u[RAT ? 2 * NV * LDH : 1];
   const int lane = threadIdx.x;
   const bool upper = lane >= 32;
   const int hh = upper ? 1 : 0;
@@ -134,6 +249,10 @@ void k_tridiag_pair(int k, int K, int64_t np, const int32_t* __restrict__ perm, 
   }
   double* taup = tau + pq * K;
   double* ep = ee + pq * K;
+  if (RAT) {   // off-diagonal entries past the last step stay 0
+    s_e[hh * LDH + r] = 0.0;
+    s_e[hh * LDH + r1] = 0.0;
+  }
   double* svh = sv + hh * LDH;
   double2* svwh = svw + hh * LDH;
 #pragma unroll
@@ -215,7 +334,11 @@ void k_tridiag_pair(int k, int K, int64_t np, const int32_t* __restrict__ perm, 
         if (r > j && r < k) Bw[(int64_t)j * k + r] = v0;
         if (r1 > j && r1 < k) Bw[(int64_t)j * k + r1] = v1;
       }
-      if (act && r == 0) { taup[j] = t; ep[j] = nz ? beta : alpha; }
+      if (act && r == 0) {
+        taup[j] = t;
+        if (!RAT) ep[j] = nz ? beta : alpha;
+      }
+      if (RAT && r == 0) s_e[hh * LDH + j] = nz ? beta : alpha;
       __syncthreads();
     }
     // keep the scheduler from interleaving steps (register pressure)
@@ -227,22 +350,34 @@ void k_tridiag_pair(int k, int K, int64_t np, const int32_t* __restrict__ perm, 
     dg0 = (c == r) ? a0[c] : dg0;
     dg1 = (c == r1) ? a1[c] : dg1;
   }
+  if constexpr (RAT) {
+    rat_tail<KQ, NV>(k, K, act, pq, nsl, r, upper, dg0, dg1, xv0, xv1, s_d, s_e, s_u, U, rtab,
+                     rtabn, rnb);
+  }
   if (!act) return;
   if (r < K) {
-    dd[pq * K + r] = r < k ? dg0 : 0.0;
-    if (r >= k - 1) { ep[r] = 0.0; taup[r] = 0.0; }
+    if (!RAT) dd[pq * K + r] = r < k ? dg0 : 0.0;
+    if (r >= k - 1) {
+      if (!RAT) ep[r] = 0.0;
+      taup[r] = 0.0;
+    }
   }
   if (r1 < K) {
-    dd[pq * K + r1] = r1 < k ? dg1 : 0.0;
-    if (r1 >= k - 1) { ep[r1] = 0.0; taup[r1] = 0.0; }
+    if (!RAT) dd[pq * K + r1] = r1 < k ? dg1 : 0.0;
+    if (r1 >= k - 1) {
+      if (!RAT) ep[r1] = 0.0;
+      taup[r1] = 0.0;
+    }
   }
-  if (NV > 0) {
+  if (NV > 0 && !RAT) {
     double* Up = U + pq * (int64_t)NV * K;
 #pragma unroll
     for (int qv = 0; qv < NV; ++qv) {
       if (r < K) Up[qv * K + r] = xv0[qv];
       if (r1 < K) Up[qv * K + r1] = xv1[qv];
     }
+  }
+  if (NV > 0) {
     if (r == 0) {
       nvec[pq] = nsl;
       status[pq] = nsl > NV - 1 ? 3 : 0;   // more slabs than carried: Jacobi
@@ -253,17 +388,22 @@ void k_tridiag_pair(int k, int K, int64_t np, const int32_t* __restrict__ perm, 
 }
 
 template <int KQ>
-bool launch_tridiag_pair_kq(hipStream_t st, int64_t np, const int32_t* perm, int NV, int k,
-                            int K, const LocDev& L, const StateDev& S, const VarsDev& V,
-                            const ChunkDev& C, const DiagDev& D, double* B, const double* bvec,
-                            double* tau, double* dd, double* ee, double* U, int* nvec,
-                            int* status) {
+int launch_tridiag_pair_kq(hipStream_t st, int64_t np, const int32_t* perm, int NV, int k, int K,
+                           const LocDev& L, const StateDev& S, const VarsDev& V,
+                           const ChunkDev& C, const DiagDev& D, double* B, const double* bvec,
+                           double* tau, double* dd, double* ee, double* U, int* nvec,
+                           int* status, const double* rtab, const int* rtabn, int rnb) {
   const unsigned nb = (unsigned)((np + 1) / 2);
+  const bool rat = rnb > 0;
   switch (NV) {
-    case 0: k_tridiag_pair<KQ, 0><<<nb, 64, 0, st>>>(k, K, np, perm, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status); return true;
-    case 2: k_tridiag_pair<KQ, 2><<<nb, 64, 0, st>>>(k, K, np, perm, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status); return true;
-    case 3: k_tridiag_pair<KQ, 3><<<nb, 64, 0, st>>>(k, K, np, perm, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status); return true;
-    default: return false;   // more carried vectors: too few registers for two problems
+    case 0: k_tridiag_pair<KQ, 0, false><<<nb, 64, 0, st>>>(k, K, np, perm, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, rtab, rtabn, rnb); return 1;
+    case 2:
+      if (rat) { k_tridiag_pair<KQ, 2, true><<<nb, 64, 0, st>>>(k, K, np, perm, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, rtab, rtabn, rnb); return 2; }
+      k_tridiag_pair<KQ, 2, false><<<nb, 64, 0, st>>>(k, K, np, perm, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, rtab, rtabn, rnb); return 1;
+    case 3:
+      if (rat) { k_tridiag_pair<KQ, 3, true><<<nb, 64, 0, st>>>(k, K, np, perm, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, rtab, rtabn, rnb); return 2; }
+      k_tridiag_pair<KQ, 3, false><<<nb, 64, 0, st>>>(k, K, np, perm, L, S, V, C, D, B, bvec, tau, dd, ee, U, nvec, status, rtab, rtabn, rnb); return 1;
+    default: return 0;   // more carried vectors: too few registers for two problems
   }
 }
 
