@@ -408,11 +408,15 @@ def bench_c4(args):
         den = np.abs(b).max()
         return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
 
-    def run(prec, path, steps):
+    def run(prec, path, steps, tune=False):
         tu = time.perf_counter()
         s = LetkfGpuSolver(prob, precision=prec, path=path)
         up = time.perf_counter() - tu
         try:
+            tuning = None
+            if tune and not args.no_autotune:
+                tuning = s.autotune(repeats=1)
+                log(f"[c4] autotune: {tuning}")
             s.run()
             t1 = time.perf_counter()
             for _ in range(steps):
@@ -435,10 +439,12 @@ def bench_c4(args):
                              "finalize": round(tm["finalize_ms"], 2)},
                "work": {"solves": int(tm["n_problems"]), "kept_pairs": int(tm["n_pairs"])},
                "error_vs_fp64_oracle": err}
+        if tuning:
+            res["tuning"] = tuning
         log(f"[c4] {path} {prec}: {json.dumps(res)}")
         return res, tm
 
-    main_res, tm = run(args.precision, "batched", args.steps)
+    main_res, tm = run(args.precision, "batched", args.steps, tune=True)
     out = {"metric": "analysis gridpoints/sec, 0.5deg 75-level T/S/u/v, k=100, batched "
                      "mixed-prec vs FP64 GPU; rel err vs FP64 oracle",
            "value": main_res["gridpoints_per_s"], "unit": "gridpoints/s", "n_gpus": 1,
@@ -448,7 +454,8 @@ def bench_c4(args):
                                   "(13,000 profiles x 75 + 25,000 surface), LOC_NOVRT 700->200 km "
                                   "cut-off, RTPS 0.6", "n_gp": prob.nij, "ns": prob.ns,
                       "k": prob.k, "n_obs": prob.nobs, "parallelism": "single GPU"},
-           "roofline": roofline(tm, prob.k, "batched", args.precision, "c4", None, prob.ns,
+           "roofline": roofline(tm, prob.k, "batched", args.precision, "c4",
+                                main_res.get("tuning"), prob.ns,
                                 prob.ns, prob.nij, prob.nobs),
            "batched": main_res, "oracle_sample": int(pts.size)}
     if args.no_fp64_compare:
@@ -456,7 +463,7 @@ def bench_c4(args):
         print(json.dumps(out), flush=True)
         return
     others = {}
-    for prec in ("fp32", "bf16x3", "bf16"):
+    for prec in ("fp32", "bf16x3", "bf16x4", "bf16"):
         if prec != args.precision:
             others[prec], _ = run(prec, "batched", 1)
     plain, _ = run("fp64", "plain", max(1, min(args.steps, 2)))
@@ -476,7 +483,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3")
     ap.add_argument("--path", default="batched", choices=["plain", "batched"])
-    ap.add_argument("--precision", default="bf16x4", choices=list(PEAK))
+    ap.add_argument("--precision", default=None, choices=list(PEAK),
+                    help="batched products: bf16x4 by default; bf16x3 for c4, the precision "
+                         "that configuration names")
     ap.add_argument("--no-alternatives", action="store_true",
                     help="skip timing the other split-BF16 product (bf16x3 / bf16x4) beside it")
     ap.add_argument("--no-fp64-compare", action="store_true",
@@ -501,6 +510,8 @@ def main():
                     help="c5: gridpoints per cycle checked against the FP64 oracle; "
                          "c4: gridpoints checked")
     args = ap.parse_args()
+    if args.precision is None:
+        args.precision = "bf16x3" if args.config == "c4" else "bf16x4"
     tun_fixed = None
     if args.tuning:
         v = [int(x) for x in args.tuning.split(",")]

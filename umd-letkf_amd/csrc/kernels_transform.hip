@@ -26,6 +26,14 @@ namespace lk {
                              double* B, const double* bvec, const double* tau);
 LK_NR(40) LK_NR(48) LK_NR(56) LK_NR(64)
 #undef LK_NR
+// k_blk_tridiag / k_blk_back launchers (one file per block size)
+#define LK_BS(n)                                                                          \
+  void launch_blk_tridiag_bs##n(hipStream_t st, unsigned nb, int k, const ChunkDev& C,    \
+                                double* B, double* tau, double* dd, double* ee);          \
+  void launch_blk_back_bs##n(hipStream_t st, unsigned nb, int k, const ChunkDev& C,       \
+                             double* B, const double* bvec, const double* tau);
+LK_BS(4) LK_BS(5) LK_BS(6) LK_BS(7) LK_BS(8)
+#undef LK_BS
 
 // M below the diagonal, one thread per column c; the rows are visited in the
 // same order by every lane (x starts at g_c in row c, is 0 above it), so the
@@ -46,14 +54,23 @@ __device__ __forceinline__ void mat_rational_rows(int k, int c, int rfirst, int 
     x[n] = 0.0;
   }
   for (int r = rfirst; r < k; ++r) {
-    double acc = 0.0;
+    // four partial sums: the node chains are independent, the sum need not
+    // be one dependent chain
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
       x[n] = r == c ? g[n] : -Us[n][r] * x[n];
-      acc = fma(cf[n], x[n], acc);
+      acc[n & 3] = fma(cf[n], x[n], acc[n & 3]);
     }
-    if (r >= c) Bp[(int64_t)r * k + c] = acc;
+    if (r >= c) Bp[(int64_t)r * k + c] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   }
+}
+
+// 1/x: the hardware estimate and one Newton step, which squares its
+// relative error (as the vector path's rational solve)
+__device__ __forceinline__ double mat_rcp(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  return y * (2.0 - x * y);
 }
 __global__ __launch_bounds__(128) void k_mat_rational(int k, ChunkDev C, double* __restrict__ B,
                                                       const double* __restrict__ dd,
@@ -87,30 +104,36 @@ __global__ __launch_bounds__(128) void k_mat_rational(int k, ChunkDev C, double*
   const double* wn = sn + 32;
   // unused node rows stay 0, so that their chains stay 0
   for (int n = nn; n < kMaxNodes; ++n) { Us[n][tid] = 0.0; Gs[n][tid] = 0.0; }
-  // bottom-up pivots D- and U_i = e_(i-1) / D-_i (wave 1, one lane per node)
-  if (tid >= 64 && tid - 64 < nn) {
-    const int n = tid - 64;
-    const double sg = m * sn[n];
-    double Up = 0.0;
-    for (int i = k - 1; i >= 0; --i) {
-      const double Dm = sd[i] + sg - (i < k - 1 ? se[i] * Up : 0.0);
-      const double Ui = i > 0 ? se[i - 1] / Dm : 0.0;
-      Us[n][i] = Ui;
-      Gs[n][i] = Dm;
-      Up = Ui;
-    }
-  }
-  __syncthreads();
-  // top-down pivots D+ and the diagonal 1 / g_i of each inverse (wave 0)
+  // one lane per node runs both pivot recurrences in one loop, step s
+  // taking the top-down pivot D+ of row s and the bottom-up pivot D- of row
+  // k-1-s (two independent chains).  U_i = e_(i-1) / D-_i goes to Us; the
+  // diagonal of the inverse, 1 / g_i with g_i = D+_i + D-_i - (d_i + s),
+  // goes to Gs once both pivots of row i are known (the first of the two
+  // waits in Gs[n][i], written and read by the same lane)
   if (tid < nn) {
     const int n = tid;
     const double sg = m * sn[n];
-    double Lp = 0.0;
-    for (int i = 0; i < k; ++i) {
-      const double Dp = sd[i] + sg - (i > 0 ? se[i - 1] * Lp : 0.0);
-      const double Li = i < k - 1 ? se[i] / Dp : 0.0;
-      Gs[n][i] = 1.0 / (Dp + Gs[n][i] - (sd[i] + sg));
-      Lp = Li;
+    double Lp = 0.0, Up = 0.0;
+#pragma unroll 2
+    for (int st = 0; st < k; ++st) {
+      const int it = st, ib = k - 1 - st;
+      const double Dp = sd[it] + sg - (it > 0 ? se[it - 1] * Lp : 0.0);
+      const double Dm = sd[ib] + sg - (ib < k - 1 ? se[ib] * Up : 0.0);
+      Lp = it < k - 1 ? se[it] * mat_rcp(Dp) : 0.0;
+      const double Ui = ib > 0 ? se[ib - 1] * mat_rcp(Dm) : 0.0;
+      Us[n][ib] = Ui;
+      Up = Ui;
+      if (it < ib) {            // first half: park the pivots
+        Gs[n][it] = Dp;
+        Gs[n][ib] = Dm;
+      } else if (it == ib) {    // middle row (k odd)
+        Gs[n][it] = mat_rcp(Dp + Dm - (sd[it] + sg));
+      } else {                  // second half: the other pivot is parked
+        const double dmt = Gs[n][it];   // D- of row it
+        const double dpb = Gs[n][ib];   // D+ of row ib
+        Gs[n][it] = mat_rcp(Dp + dmt - (sd[it] + sg));
+        Gs[n][ib] = mat_rcp(dpb + Dm - (sd[ib] + sg));
+      }
     }
   }
   __syncthreads();
@@ -351,19 +374,37 @@ bool transform_mat_supported(int k) { return k >= 64 && k <= 128; }
 void launch_transform_mat(hipStream_t st, int prec, const LocDev& L, const StateDev& S,
                           const VarsDev& V, const ChunkDev& C, const DiagDev& D, double* B,
                           const double* bvec, double* tau, double* dd, double* ee,
-                          const RatTab& R, double* ana) {
+                          const RatTab& R, double* ana, bool blk) {
   const int64_t np = C.ngp * C.G;
   if (np <= 0) return;
   const int k = (int)S.k;
   const unsigned nb = (unsigned)np;
-  // rows per half: k <= 2 NR
+  // blk: the 2D-block kernels, BS x BS blocks with 16 BS >= k; else the
+  // column-half kernels, rows per half k <= 2 NR
+  const int bs = (k + 15) / 16;
   const int nr = (k + 1) / 2;
-  if (nr <= 40) launch_mat_tridiag_nr40(st, nb, k, C, B, tau, dd, ee);
+  if (blk) {
+    switch (bs) {
+      case 4: launch_blk_tridiag_bs4(st, nb, k, C, B, tau, dd, ee); break;
+      case 5: launch_blk_tridiag_bs5(st, nb, k, C, B, tau, dd, ee); break;
+      case 6: launch_blk_tridiag_bs6(st, nb, k, C, B, tau, dd, ee); break;
+      case 7: launch_blk_tridiag_bs7(st, nb, k, C, B, tau, dd, ee); break;
+      default: launch_blk_tridiag_bs8(st, nb, k, C, B, tau, dd, ee); break;
+    }
+  } else if (nr <= 40) launch_mat_tridiag_nr40(st, nb, k, C, B, tau, dd, ee);
   else if (nr <= 48) launch_mat_tridiag_nr48(st, nb, k, C, B, tau, dd, ee);
   else if (nr <= 56) launch_mat_tridiag_nr56(st, nb, k, C, B, tau, dd, ee);
   else launch_mat_tridiag_nr64(st, nb, k, C, B, tau, dd, ee);
   k_mat_rational<<<nb, 128, 0, st>>>(k, C, B, dd, ee, R.tab, R.n, R.nb);
-  if (nr <= 40) launch_mat_back_nr40(st, nb, k, C, B, bvec, tau);
+  if (blk) {
+    switch (bs) {
+      case 4: launch_blk_back_bs4(st, nb, k, C, B, bvec, tau); break;
+      case 5: launch_blk_back_bs5(st, nb, k, C, B, bvec, tau); break;
+      case 6: launch_blk_back_bs6(st, nb, k, C, B, bvec, tau); break;
+      case 7: launch_blk_back_bs7(st, nb, k, C, B, bvec, tau); break;
+      default: launch_blk_back_bs8(st, nb, k, C, B, bvec, tau); break;
+    }
+  } else if (nr <= 40) launch_mat_back_nr40(st, nb, k, C, B, bvec, tau);
   else if (nr <= 48) launch_mat_back_nr48(st, nb, k, C, B, bvec, tau);
   else if (nr <= 56) launch_mat_back_nr56(st, nb, k, C, B, bvec, tau);
   else launch_mat_back_nr64(st, nb, k, C, B, bvec, tau);

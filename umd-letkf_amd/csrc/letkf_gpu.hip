@@ -153,6 +153,7 @@ struct letkf_gpu_t {
   int tune_chunk_log2 = 20;   // at most 2^n problems per chunk (17..20)
   int tune_vec = 1;       // vector path when every group has few slabs
   int tune_fuse = 1;      // rational solve inside the two-problem tridiagonalization
+  int tune_matblk = 1;    // 64 <= k <= 128: 2D-block reduction and back-transform
   int tune_rat = 1;       // vector path: rational A^(-1/2), no eigensolve (1 batched
                           // path only, 2 the plain FP64 path too, 0 off)
   int tune_mat = 1;       // batched, 64 <= k <= 128: explicit T applied on MFMA
@@ -302,6 +303,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   if (const char* e = getenv("LETKF_GPU_FOLD")) h->fold_log2 = std::min(10, std::max(5, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_TILE")) h->tune_tile = atoi(e) != 0;
   if (const char* e = getenv("LETKF_GPU_FUSE")) h->tune_fuse = atoi(e) != 0;
+  if (const char* e = getenv("LETKF_GPU_MATBLK")) h->tune_matblk = atoi(e) != 0;
   if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = std::min(3, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_BBITS")) h->tune_bbits = std::min(2, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_TRIREG")) h->tune_trireg = std::min(2, std::max(0, atoi(e)));
@@ -987,7 +989,8 @@ static hipError_t enqueue_solve(letkf_gpu_t* h, hipStream_t st, const RunViews& 
   if (h->use_mat) {
     const RatTab R{h->d_rat, h->d_ratn, kRatBuckets};
     launch_transform_mat(st, h->batched() ? h->dims.precision : LETKF_PREC_FP64, h->loc, W.S,
-                         W.V, C, W.D, h->d_B, h->d_b, h->d_tau, h->d_dd, h->d_ee, R, h->d_ana);
+                         W.V, C, W.D, h->d_B, h->d_b, h->d_tau, h->d_dd, h->d_ee, R, h->d_ana,
+                         h->tune_matblk != 0);
   } else if (h->eigK) {
     RatTab R;
     if (h->use_rat) { R.tab = h->d_rat; R.n = h->d_ratn; R.nb = kRatBuckets; }

@@ -249,7 +249,7 @@ bool transform_mat_supported(int k);
 void launch_transform_mat(hipStream_t st, int prec, const LocDev& L, const StateDev& S,
                           const VarsDev& V, const ChunkDev& C, const DiagDev& D, double* B,
                           const double* bvec, double* tau, double* dd, double* ee,
-                          const RatTab& R, double* ana);
+                          const RatTab& R, double* ana, bool blk);
 void launch_finalize(hipStream_t st, const LocDev& L, const StateDev& S,
                      const VarsDev& V, const ChunkDev& C, const DiagDev& D,
                      float rtps, float rtpp, float mul, const double* ana_in,

@@ -1,0 +1,319 @@
+// transform_blk.h -- the reduction and the back-transform of the explicit
+// transform matrix (64 <= k <= 128, see transform_mat.h) with A distributed
+// over a 256-thread block in 2D: the matrix, padded to N = 16 BS, is a 16 x 16
+// grid of BS x BS blocks, and thread t holds block (I, J) = (t / 16, t % 16)
+// in registers.  Per step a thread needs v and w only for its BS rows and
+// its BS columns (4 BS values from LDS), where the column layout of
+// transform_mat.h needs v and w for all of its NR rows, broadcast to every
+// thread of the block: the LDS traffic per step falls from about 24 NR to
+// about 4 BS doubles per thread.  The symmetric product's row sums are DPP
+// sums over the 16 threads of a grid row (consecutive lanes, one DPP row).
+// Two barriers per step: after the reflector's column (or v) is in LDS, and
+// after p = t A v and the partial sums of v.p are.
+//
+//  k_blk_tridiag<BS>  A = Q Tri Q^T (LAPACK dsytd2, lower), the outputs of
+//                     k_mat_tridiag: reflector j in row j of B (columns > j),
+//                     tau, d and e.  Waves whose rows are all reduced skip
+//                     the arithmetic of a step.
+//  k_blk_back<BS>     W = Q M Q^T from M in the lower triangle of B, then
+//                     wbar = W (W b) / (k-1) and T = W + 1 wbar^T over B, the
+//                     outputs of k_mat_back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "eigen_common.h"
+#include "letkf_internal.h"
+
+namespace lk {
+
+constexpr int kBlkStride = 128;   // d, e, tau entries per problem (as kMatStride)
+
+// DPP move of a double with no old value: lanes without a source read 0
+template <int CTRL>
+__device__ __forceinline__ double blk_dpp(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
+// sum over the 16 lanes of this lane's DPP row, complete in the row's lane 15
+__device__ __forceinline__ double row16_sum(double x) {
+  x += blk_dpp<0x111>(x);
+  x += blk_dpp<0x112>(x);
+  x += blk_dpp<0x114>(x);
+  x += blk_dpp<0x118>(x);
+  return x;
+}
+
+template <int BS>
+__global__ __launch_bounds__(256) void k_blk_tridiag(int k, ChunkDev C, double* __restrict__ B,
+                                                     double* __restrict__ tau,
+                                                     double* __restrict__ dd,
+                                                     double* __restrict__ ee) {
+  static_assert(16 * BS <= 128, "at most 128 rows");
+  __shared__ double sx[128];        // column j of A
+  __shared__ double sp[128];        // p = t A v
+  __shared__ double skc[4];         // per-wave sums of v . p
+  __shared__ double se[128], stau[128];
+  const int64_t pair = blockIdx.x;
+  if (!C.solve[pair]) return;
+  const int tid = threadIdx.x, l = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int I = tid >> 4, J = tid & 15;
+  const int R0 = BS * I, C0 = BS * J;
+  const double m = (double)(k - 1);
+  double* Bp = B + pair * (int64_t)k * k;
+  double a[BS][BS];
+#pragma unroll
+  for (int x = 0; x < BS; ++x)
+#pragma unroll
+    for (int y = 0; y < BS; ++y) {
+      const int r = R0 + x, c = C0 + y;
+      double v = 0.0;
+      if (r < k && c < k) {
+        v = Bp[(int64_t)r * k + c];
+        if (r == c) v += m;
+      }
+      a[x][y] = v;
+    }
+  if (tid < 128) { sx[tid] = 0.0; sp[tid] = 0.0; se[tid] = 0.0; stau[tid] = 0.0; }
+  __syncthreads();
+  const int jend = k - 2;
+  // last row of this wave's grid rows (wave-uniform)
+  const int wlast = 4 * BS * wv + 4 * BS - 1;
+  for (int jb = 0; jb * BS < jend; ++jb) {
+#pragma unroll
+    for (int bj = 0; bj < BS; ++bj) {
+      const int j = jb * BS + bj;
+      if (j >= jend) break;
+      const int j1 = j + 1;
+      // column j (local column bj of grid column jb) into LDS
+      if (J == jb) {
+#pragma unroll
+        for (int x = 0; x < BS; ++x) sx[R0 + x] = a[x][bj];
+      }
+      __syncthreads();
+      // the reflector, the same in every wave
+      const double x0 = sx[l], x1 = sx[l + 64];
+      double q = 0.0;
+      if (l > j1 && l < k) q += x0 * x0;
+      if (l + 64 > j1 && l + 64 < k) q += x1 * x1;
+      const double sig = wave_sum_dpp(q);
+      const double alpha = sx[j1];
+      // column already reduced (sig = 0): tau = 0, and the step changes nothing
+      const bool refl = sig != 0.0;
+      const double mu = sqrt(alpha * alpha + sig);
+      const double beta = refl ? (alpha >= 0.0 ? -mu : mu) : alpha;
+      const double t = refl ? (beta - alpha) / beta : 0.0;
+      const double sc = refl ? 1.0 / (alpha - beta) : 0.0;
+      double vr[BS], vc[BS];
+#pragma unroll
+      for (int x = 0; x < BS; ++x) {
+        const int r = R0 + x, c = C0 + x;
+        vr[x] = r == j1 ? 1.0 : (r > j1 ? sx[r] * sc : 0.0);
+        vc[x] = c == j1 ? 1.0 : (c > j1 ? sx[c] * sc : 0.0);
+      }
+      const bool live = wlast > j;   // this wave still has rows below j
+      if (live) {
+        // row sums of this block's part of A v, completed over the grid row
+        double P[BS];
+#pragma unroll
+        for (int x = 0; x < BS; ++x) {
+          double s0 = 0.0;
+#pragma unroll
+          for (int y = 0; y < BS; ++y) s0 = fma(a[x][y], vc[y], s0);
+          P[x] = row16_sum(s0);
+        }
+        double part = 0.0;
+        if (J == 15) {
+#pragma unroll
+          for (int x = 0; x < BS; ++x) {
+            const double px = R0 + x > j ? t * P[x] : 0.0;
+            sp[R0 + x] = px;
+            part = fma(px, vr[x], part);
+          }
+        }
+        const double ws = wave_sum_dpp(part);
+        if (l == 0) skc[wv] = ws;
+      } else if (l == 0) {
+        skc[wv] = 0.0;
+      }
+      // reflector j (columns > j) and tau, e
+      if (I == 0) {
+#pragma unroll
+        for (int y = 0; y < BS; ++y) {
+          const int c = C0 + y;
+          if (c > j && c < k) Bp[(int64_t)j * k + c] = vc[y];
+        }
+      }
+      if (tid == 0) { stau[j] = t; se[j] = beta; }
+      __syncthreads();
+      if (live) {
+        const double kc = 0.5 * t * ((skc[0] + skc[1]) + (skc[2] + skc[3]));
+        double wr[BS], wc[BS];
+#pragma unroll
+        for (int x = 0; x < BS; ++x) {
+          const int r = R0 + x, c = C0 + x;
+          wr[x] = r > j ? sp[r] - kc * vr[x] : 0.0;
+          wc[x] = c > j ? sp[c] - kc * vc[x] : 0.0;
+        }
+#pragma unroll
+        for (int x = 0; x < BS; ++x)
+#pragma unroll
+          for (int y = 0; y < BS; ++y) a[x][y] = fma(-vr[x], wc[y], fma(-wr[x], vc[y], a[x][y]));
+      }
+    }
+  }
+  __syncthreads();
+  double* dp = dd + pair * kBlkStride;
+  double* ep = ee + pair * kBlkStride;
+  if (I == J) {
+#pragma unroll
+    for (int x = 0; x < BS; ++x)
+      if (R0 + x < k) dp[R0 + x] = a[x][x];
+  }
+  // the last subdiagonal entry A[k-1][k-2]
+#pragma unroll
+  for (int x = 0; x < BS; ++x)
+#pragma unroll
+    for (int y = 0; y < BS; ++y)
+      if (R0 + x == k - 1 && C0 + y == k - 2) ep[k - 2] = a[x][y];
+  if (tid < jend) {
+    ep[tid] = se[tid];
+    tau[pair * kBlkStride + tid] = stau[tid];
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(256) void k_blk_back(int k, ChunkDev C, double* __restrict__ B,
+                                                  const double* __restrict__ bvec,
+                                                  const double* __restrict__ tau) {
+  static_assert(16 * BS <= 128, "at most 128 rows");
+  __shared__ double sv[2][128];     // reflector of the step (double buffered)
+  __shared__ double sp[128];        // p = t A v; then W b, W (W b)
+  __shared__ double skc[4];
+  __shared__ double stau[128];
+  const int64_t pair = blockIdx.x;
+  if (!C.solve[pair]) return;
+  const int tid = threadIdx.x, l = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int I = tid >> 4, J = tid & 15;
+  const int R0 = BS * I, C0 = BS * J;
+  const double m = (double)(k - 1);
+  double* Bp = B + pair * (int64_t)k * k;
+  // M from the lower triangle of B (symmetric)
+  double a[BS][BS];
+#pragma unroll
+  for (int x = 0; x < BS; ++x)
+#pragma unroll
+    for (int y = 0; y < BS; ++y) {
+      const int r = R0 + x, c = C0 + y;
+      double v = 0.0;
+      if (r < k && c < k) v = r >= c ? Bp[(int64_t)r * k + c] : Bp[(int64_t)c * k + r];
+      a[x][y] = v;
+    }
+  if (tid < 128) {
+    stau[tid] = tid < k - 2 ? tau[pair * kBlkStride + tid] : 0.0;
+    sv[0][tid] = 0.0;
+    sv[1][tid] = 0.0;
+    sp[tid] = 0.0;
+  }
+  // reflector rows go through LDS: threads 0..127 hold entry tid of the next
+  // one (loaded a step ahead)
+  double vnext = 0.0;
+  if (k - 3 >= 0 && tid < 128) {
+    const int j = k - 3;
+    vnext = (tid > j && tid < k) ? Bp[(int64_t)j * k + tid] : 0.0;
+  }
+  __syncthreads();
+  for (int j = k - 3; j >= 0; --j) {
+    const int buf = j & 1;
+    if (tid < 128) sv[buf][tid] = vnext;
+    if (tid < 128 && j > 0) {
+      const int jn = j - 1;
+      vnext = (tid > jn && tid < k) ? Bp[(int64_t)jn * k + tid] : 0.0;
+    }
+    __syncthreads();
+    const double t = stau[j];
+    double vr[BS], vc[BS];
+#pragma unroll
+    for (int x = 0; x < BS; ++x) {
+      vr[x] = sv[buf][R0 + x];
+      vc[x] = sv[buf][C0 + x];
+    }
+    double P[BS];
+#pragma unroll
+    for (int x = 0; x < BS; ++x) {
+      double s0 = 0.0;
+#pragma unroll
+      for (int y = 0; y < BS; ++y) s0 = fma(a[x][y], vc[y], s0);
+      P[x] = row16_sum(s0);
+    }
+    double part = 0.0;
+    if (J == 15) {
+#pragma unroll
+      for (int x = 0; x < BS; ++x) {
+        const double px = t * P[x];
+        sp[R0 + x] = px;
+        part = fma(px, vr[x], part);
+      }
+    }
+    const double ws = wave_sum_dpp(part);
+    if (l == 0) skc[wv] = ws;
+    __syncthreads();
+    const double kc = 0.5 * t * ((skc[0] + skc[1]) + (skc[2] + skc[3]));
+    double wr[BS], wc[BS];
+#pragma unroll
+    for (int x = 0; x < BS; ++x) {
+      wr[x] = sp[R0 + x] - kc * vr[x];
+      wc[x] = sp[C0 + x] - kc * vc[x];
+    }
+#pragma unroll
+    for (int x = 0; x < BS; ++x)
+#pragma unroll
+      for (int y = 0; y < BS; ++y) a[x][y] = fma(-vr[x], wc[y], fma(-wr[x], vc[y], a[x][y]));
+  }
+  // wbar = W (W b) / (k-1); T = W + 1 wbar^T, row major over B
+  __syncthreads();
+  if (tid < 128) sv[0][tid] = tid < k ? bvec[pair * (int64_t)k + tid] : 0.0;
+  __syncthreads();
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    double vc[BS];
+#pragma unroll
+    for (int y = 0; y < BS; ++y) vc[y] = sv[pass][C0 + y];
+#pragma unroll
+    for (int x = 0; x < BS; ++x) {
+      double s0 = 0.0;
+#pragma unroll
+      for (int y = 0; y < BS; ++y) s0 = fma(a[x][y], vc[y], s0);
+      s0 = row16_sum(s0);
+      if (J == 15) {
+        if (pass == 0) sv[1][R0 + x] = s0;
+        else sp[R0 + x] = s0 / m;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < BS; ++x)
+#pragma unroll
+    for (int y = 0; y < BS; ++y) {
+      const int r = R0 + x, c = C0 + y;
+      if (r < k && c < k) Bp[(int64_t)r * k + c] = a[x][y] + sp[c];
+    }
+}
+
+template <int BS>
+void launch_blk_tridiag_bs(hipStream_t st, unsigned nb, int k, const ChunkDev& C, double* B,
+                           double* tau, double* dd, double* ee) {
+  k_blk_tridiag<BS><<<nb, 256, 0, st>>>(k, C, B, tau, dd, ee);
+}
+template <int BS>
+void launch_blk_back_bs(hipStream_t st, unsigned nb, int k, const ChunkDev& C, double* B,
+                        const double* bvec, const double* tau) {
+  k_blk_back<BS><<<nb, 256, 0, st>>>(k, C, B, bvec, tau);
+}
+
+}  // namespace lk
