@@ -414,7 +414,11 @@ def bench_c4(args):
         up = time.perf_counter() - tu
         try:
             tuning = None
-            if tune and not args.no_autotune:
+            if tune and args.tun_fixed:
+                s.set_tuning(**args.tun_fixed)
+                tuning = s.tuning()
+                log(f"[c4] tuning set: {tuning}")
+            elif tune and not args.no_autotune:
                 tuning = s.autotune(repeats=1)
                 log(f"[c4] autotune: {tuning}")
             s.run()
@@ -518,6 +522,7 @@ def main():
         v += [1, 0, 20, 2, 0][len(v) - 2:] if len(v) < 7 else []
         tun_fixed = dict(order=v[0], bucket_bits=v[1], tridiag_reg=v[2], overlap=v[3],
                          chunk_log2=v[4], xcd=v[5], tile=v[6])
+    args.tun_fixed = tun_fixed
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, args.config, args.path, args.precision,
                       args.problems_per_dispatch, tun_fixed)

@@ -64,18 +64,23 @@ __global__ __launch_bounds__(256) void k_blk_tridiag(int k, ChunkDev C, double* 
   const int R0 = BS * I, C0 = BS * J;
   const double m = (double)(k - 1);
   double* Bp = B + pair * (int64_t)k * k;
+  // every load is issued (clamped index) before any value is used, so the
+  // loads are in flight together; padding rows and columns are zeroed after
   double a[BS][BS];
 #pragma unroll
   for (int x = 0; x < BS; ++x)
 #pragma unroll
     for (int y = 0; y < BS; ++y) {
+      const int r = min(R0 + x, k - 1), c = min(C0 + y, k - 1);
+      a[x][y] = Bp[(int64_t)r * k + c];
+    }
+#pragma unroll
+  for (int x = 0; x < BS; ++x)
+#pragma unroll
+    for (int y = 0; y < BS; ++y) {
       const int r = R0 + x, c = C0 + y;
-      double v = 0.0;
-      if (r < k && c < k) {
-        v = Bp[(int64_t)r * k + c];
-        if (r == c) v += m;
-      }
-      a[x][y] = v;
+      const double v = (r < k && c < k) ? a[x][y] : 0.0;
+      a[x][y] = (r == c && r < k) ? v + m : v;
     }
   if (tid < 128) { sx[tid] = 0.0; sp[tid] = 0.0; se[tid] = 0.0; stau[tid] = 0.0; }
   __syncthreads();
@@ -107,12 +112,16 @@ __global__ __launch_bounds__(256) void k_blk_tridiag(int k, ChunkDev C, double* 
       const double beta = refl ? (alpha >= 0.0 ? -mu : mu) : alpha;
       const double t = refl ? (beta - alpha) / beta : 0.0;
       const double sc = refl ? 1.0 / (alpha - beta) : 0.0;
+      // branch-free: the LDS reads are unconditional and the row tests only
+      // pick factors (a conditional read becomes a branch and a full LDS
+      // wait per element)
       double vr[BS], vc[BS];
 #pragma unroll
       for (int x = 0; x < BS; ++x) {
         const int r = R0 + x, c = C0 + x;
-        vr[x] = r == j1 ? 1.0 : (r > j1 ? sx[r] * sc : 0.0);
-        vc[x] = c == j1 ? 1.0 : (c > j1 ? sx[c] * sc : 0.0);
+        const double xr = sx[r], xc = sx[c];
+        vr[x] = fma(xr, r > j1 ? sc : 0.0, r == j1 ? 1.0 : 0.0);
+        vc[x] = fma(xc, c > j1 ? sc : 0.0, c == j1 ? 1.0 : 0.0);
       }
       const bool live = wlast > j;   // this wave still has rows below j
       if (live) {
@@ -155,8 +164,9 @@ __global__ __launch_bounds__(256) void k_blk_tridiag(int k, ChunkDev C, double* 
 #pragma unroll
         for (int x = 0; x < BS; ++x) {
           const int r = R0 + x, c = C0 + x;
-          wr[x] = r > j ? sp[r] - kc * vr[x] : 0.0;
-          wc[x] = c > j ? sp[c] - kc * vc[x] : 0.0;
+          const double pr = sp[r], pc = sp[c];
+          wr[x] = r > j ? pr - kc * vr[x] : 0.0;
+          wc[x] = c > j ? pc - kc * vc[x] : 0.0;
         }
 #pragma unroll
         for (int x = 0; x < BS; ++x)
@@ -202,17 +212,20 @@ __global__ __launch_bounds__(256) void k_blk_back(int k, ChunkDev C, double* __r
   const int R0 = BS * I, C0 = BS * J;
   const double m = (double)(k - 1);
   double* Bp = B + pair * (int64_t)k * k;
-  // M from the lower triangle of B (symmetric)
+  // M from the lower triangle of B (symmetric); all loads issued first
   double a[BS][BS];
 #pragma unroll
   for (int x = 0; x < BS; ++x)
 #pragma unroll
     for (int y = 0; y < BS; ++y) {
-      const int r = R0 + x, c = C0 + y;
-      double v = 0.0;
-      if (r < k && c < k) v = r >= c ? Bp[(int64_t)r * k + c] : Bp[(int64_t)c * k + r];
-      a[x][y] = v;
+      const int r = min(R0 + x, k - 1), c = min(C0 + y, k - 1);
+      a[x][y] = Bp[r >= c ? (int64_t)r * k + c : (int64_t)c * k + r];
     }
+#pragma unroll
+  for (int x = 0; x < BS; ++x)
+#pragma unroll
+    for (int y = 0; y < BS; ++y)
+      if (R0 + x >= k || C0 + y >= k) a[x][y] = 0.0;
   if (tid < 128) {
     stau[tid] = tid < k - 2 ? tau[pair * kBlkStride + tid] : 0.0;
     sv[0][tid] = 0.0;
