@@ -69,7 +69,7 @@ def product_kernel(path, precision, tun=None, k=None):
     it exists (BF16 and split-BF16 products, k <= 111), else the 32x32 one."""
     if path == "plain":
         return "k_product_f64"
-    t16 = (tun or {}).get("tile", 0) and precision != "fp32" and (
+    t16 = (tun or {}).get("tile", 1) and precision != "fp32" and (
         k is None or (k + 1 + 15) // 16 <= 7)
     return "k_product_t16" if t16 else "k_product_mx"
 
@@ -77,7 +77,7 @@ def product_kernel(path, precision, tun=None, k=None):
 def tuning_key(tun):
     if not tun:
         return "default"
-    t = {"xcd": 2, "tile": 0, **tun}
+    t = {"xcd": 2, "tile": 1, **tun}
     return "order{order}_bb{bucket_bits}_xcd{xcd}".format(**t) + ("_t16" if t["tile"] else "")
 
 
@@ -519,7 +519,7 @@ def main():
     tun_fixed = None
     if args.tuning:
         v = [int(x) for x in args.tuning.split(",")]
-        v += [1, 0, 20, 2, 0][len(v) - 2:] if len(v) < 7 else []
+        v += [2, 0, 20, 2, 1][len(v) - 2:] if len(v) < 7 else []
         tun_fixed = dict(order=v[0], bucket_bits=v[1], tridiag_reg=v[2], overlap=v[3],
                          chunk_log2=v[4], xcd=v[5], tile=v[6])
     args.tun_fixed = tun_fixed

@@ -146,7 +146,7 @@ typedef struct letkf_gpu_timing {
  * the bucket edges are the powers of two split into 2^bucket_bits parts
  * (0..2); tridiag_reg: the Householder reduction for 32 <= k < 64, 1 one
  * problem per wave with the matrix in registers, 2 two problems per wave
- * (k <= 50; 1 above), 0 the LDS kernel; overlap: 1 searches the next chunk of
+ * (k <= 50; 1 above; the default), 0 the LDS kernel; overlap: 1 searches the next chunk of
  * gridpoints on a second stream while the current one is solved;
  * chunk_log2: at most 2^chunk_log2 (gridpoint, group) problems per chunk
  * (17..20); xcd: how the product kernel's workgroups meet the 8 XCDs (each
@@ -154,7 +154,7 @@ typedef struct letkf_gpu_timing {
  * consecutive problems, the runs of the XCDs interleaved (default 2); tile:
  * the batched product's MFMA tiles, 0 32x32 blocks over rows padded to a
  * multiple of 32, 1 16x16 tiles over rows padded to a multiple of 16
- * (BF16 and split-BF16 products, k <= 111; 0 elsewhere).  None of them
+ * (BF16 and split-BF16 products, k <= 111; 0 elsewhere; the default 1).  None of them
  * changes the analysis, except that the tile shape changes the order of
  * the FP32 accumulation (the products agree to rounding). */
 typedef struct letkf_gpu_tuning {

@@ -90,18 +90,19 @@ def test_launch_choices_and_autotune():
     from letkf_amd.solver import LetkfError, LetkfGpuSolver
     p = synth.random_problem(seed=777, k=40, ns=2, nobs=4000, rtps=0.4)
     with LetkfGpuSolver(p, precision="bf16x3", path="batched") as s:
+        s.set_tuning(2, 0, 1, tile=0)
         s.run()
         st0, mn0, sp0 = s.state_f64()
         for order in range(4):
             for bb in (range(3) if order >= 2 else [0]):
                 for xcd in (0, 1, 2, 3):
                     print("order", order, "bucket_bits", bb, "xcd", xcd, flush=True)
-                    s.set_tuning(order, bb, 1, xcd=xcd)
+                    s.set_tuning(order, bb, 1, xcd=xcd, tile=0)
                     s.run()
                     st, mn, sp = s.state_f64()
                     assert np.array_equal(st, st0) and np.array_equal(mn, mn0)
         for tr in (0, 2):
-            s.set_tuning(2, 0, tr)
+            s.set_tuning(2, 0, tr, tile=0)
             s.run()
             st, mn, sp = s.state_f64()
             assert rel(st, st0) <= 1e-12 and rel(mn, mn0) <= 1e-12, tr
@@ -125,14 +126,16 @@ def test_launch_choices_and_autotune():
                 s.set_tuning(*bad)
 
 
+@pytest.mark.parametrize("tile", ["0", "1"])
 @pytest.mark.parametrize("prec", ["bf16x3", "bf16x4", "bf16"])
 @pytest.mark.parametrize("k", [15, 20, 40, 50, 100])
-def test_tile16_products(prec, k, monkeypatch):
-    """The 16x16-tile product kernel (launch choice tile = 1, here through
-    LETKF_GPU_TILE: v_mfma_f32_16x16x32_bf16, rows padded to a multiple of
-    16, 1 to 7 row tiles) meets the same bounds against the oracle as the
-    32x32 kernel; k = 100 runs the large-ensemble transform."""
-    monkeypatch.setenv("LETKF_GPU_TILE", "1")
+def test_tile_shapes(prec, k, tile, monkeypatch):
+    """Both product kernels, chosen through LETKF_GPU_TILE: 16x16 tiles
+    (tile = 1, the default: v_mfma_f32_16x16x32_bf16, rows padded to a
+    multiple of 16, 1 to 7 row tiles) and 32x32 blocks (tile = 0), meet the
+    same bounds against the oracle; k = 100 runs the large-ensemble
+    transform."""
+    monkeypatch.setenv("LETKF_GPU_TILE", tile)
     p = synth.random_problem(seed=500 + k, k=k, ns=2, nobs=4000, rtps=0.4)
     _mixed(p, prec)
 

@@ -143,12 +143,13 @@ struct letkf_gpu_t {
   size_t live_bytes = 0;
   size_t mem_limit = 0;   // 0 = none
   int tune_xcd = 2;       // product kernels: runs of 64 problems per XCD
-  int tune_tile = 0;      // batched product: 0 32x32 MFMA blocks, 1 16x16 tiles
+  int tune_tile = 1;      // batched product: 0 32x32 MFMA blocks, 1 16x16 tiles
+                          // (where that kernel exists)
   int fold_log2 = 7;      // compensated products: partial sums folded every 2^n obs
   int tune_bucket = 2;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
   int tune_bbits = 0;     // bucket edges: powers of two split in 2^bbits parts
-  int tune_trireg = 1;    // tridiagonalization (32 <= k < 64): 0 LDS, 1 register,
-                          // 2 two problems per wave (k <= 50)
+  int tune_trireg = 2;    // tridiagonalization (32 <= k < 64): 0 LDS, 1 register,
+                          // 2 two problems per wave (k <= 50; 1 above)
   int tune_overlap = 0;   // search of the next chunk concurrent with the solve
   int tune_chunk_log2 = 20;   // at most 2^n problems per chunk (17..20)
   int tune_vec = 1;       // vector path when every group has few slabs

@@ -216,8 +216,8 @@ class LetkfGpuSolver:
         self._chk(self.lib.letkf_gpu_get_tuning(self.h, C.byref(t)))
         return {f: getattr(t, f) for f, _ in abi.Tuning._fields_}
 
-    def set_tuning(self, order: int, bucket_bits: int = 0, tridiag_reg: int = 1,
-                   overlap: int = 1, chunk_log2: int = 20, xcd: int = 2, tile: int = 0, **_):
+    def set_tuning(self, order: int, bucket_bits: int = 0, tridiag_reg: int = 2,
+                   overlap: int = 1, chunk_log2: int = 20, xcd: int = 2, tile: int = 1, **_):
         t = abi.Tuning(order, bucket_bits, tridiag_reg, 0, 0.0, 0.0, overlap, chunk_log2, xcd,
                        tile)
         self._chk(self.lib.letkf_gpu_set_tuning(self.h, C.byref(t)))
