@@ -75,10 +75,55 @@ def product_kernel(path, precision, tun=None, k=None):
 
 
 def tuning_key(tun):
+    """The launch choices that shape the product kernel's memory traffic:
+    problem order, bucket edges, XCD interleave, tile shape and (when not
+    the default 2^20) the chunk size."""
     if not tun:
         return "default"
-    t = {"xcd": 2, "tile": 1, **tun}
-    return "order{order}_bb{bucket_bits}_xcd{xcd}".format(**t) + ("_t16" if t["tile"] else "")
+    t = {"xcd": 2, "tile": 1, "chunk_log2": 20, **tun}
+    return ("order{order}_bb{bucket_bits}_xcd{xcd}".format(**t) + ("_t16" if t["tile"] else "")
+            + (f"_c{t['chunk_log2']}" if t["chunk_log2"] != 20 else ""))
+
+
+def sweep_tunings(spec):
+    """Launch choices for --product-sweep: "tuner" is every product-side
+    combination the autotuner can reach (order 0..3 with bucket edges 0..2
+    for the bucketed orders, both tile shapes, XCD interleave 0..3), else a
+    ';'-separated list of order,bucket_bits,xcd,tile."""
+    base = dict(tridiag_reg=2, overlap=0, chunk_log2=20)
+    out = []
+    if spec == "tuner":
+        for order in range(4):
+            for bb in range(3 if order >= 2 else 1):
+                for tile in (1, 0):
+                    for xcd in range(4):
+                        out.append(dict(order=order, bucket_bits=bb, xcd=xcd, tile=tile, **base))
+        return out
+    for item in spec.split(";"):
+        o, b, x, t = (int(v) for v in item.split(","))
+        out.append(dict(order=o, bucket_bits=b, xcd=x, tile=t, **base))
+    return out
+
+
+def product_sweep(args):
+    """One analysis per launch choice of the sweep, in order (for per-dispatch
+    counters of the product kernel under rocprofv3 --pmc)."""
+    from letkf_amd.solver import LetkfGpuSolver
+    prob, _ = make_problem(args.config)
+    s = LetkfGpuSolver(prob, precision=args.precision, path=args.path)
+    keys, problems = [], 0
+    try:
+        for t in sweep_tunings(args.product_sweep):
+            s.set_tuning(**t)
+            s.run()
+            tm = s.timing()
+            keys.append(tuning_key(t))
+            problems = int(tm["n_problems"])
+            log(f"[sweep] {len(keys) - 1}: {keys[-1]} product {tm['product_ms']:.2f} ms "
+                f"launches {int(tm['n_launch'])}")
+    finally:
+        s.final()
+    print(json.dumps({"sweep": keys, "problems_per_dispatch": problems}))
 
 
 def roofline(t, k, path, precision, workload, tun=None, slabs_per_problem=1.0, ns=1,
@@ -137,34 +182,51 @@ def roofline(t, k, path, precision, workload, tun=None, slabs_per_problem=1.0, n
     return roof
 
 
-def summarize_pmc(fetch_csv, write_csv, workload, path, precision, problems, tun=None):
+def summarize_pmc(fetch_csv, write_csv, workload, path, precision, problems, tun=None,
+                  sweep=None):
     """Per-dispatch HBM traffic of the product kernel from two rocprofv3
-    --pmc runs, merged into profiles/traffic.json."""
+    --pmc runs, merged into profiles/traffic.json.  With a sweep (the
+    --product-sweep list the runs made) the product dispatches, in dispatch
+    order, are one per launch choice of the list; else they are averaged
+    under one launch choice."""
     import csv
-    kern = product_kernel(path, precision, tun)
+    tuns = sweep_tunings(sweep) if sweep else [tun]
+    kerns = [product_kernel(path, precision, t) for t in tuns]
 
     def per_dispatch(fn, counter):
-        vals = [float(r["Counter_Value"]) for r in csv.DictReader(open(fn))
-                if r["Counter_Name"] == counter and kern in r["Kernel_Name"]]
-        if not vals:
-            raise SystemExit(f"no {counter} rows for {kern} in {fn}")
-        return sum(vals) / len(vals), len(vals)
-    fk, n1 = per_dispatch(fetch_csv, "FETCH_SIZE")
-    wk, n2 = per_dispatch(write_csv, "WRITE_SIZE")
-    ent = dict(kernel=kern, fetch_kib=fk, write_kib=wk, dispatches=[n1, n2],
-               bytes_per_dispatch=round(1024.0 * (fk * FETCH_SCALE + wk * WRITE_SCALE)),
-               problems_per_dispatch=problems,
-               source=f"{os.path.basename(os.path.dirname(fetch_csv))}, "
-                      f"{os.path.basename(os.path.dirname(write_csv))}")
+        rows = [r for r in csv.DictReader(open(fn))
+                if r["Counter_Name"] == counter and any(kn in r["Kernel_Name"] for kn in kerns)]
+        rows.sort(key=lambda r: int(r.get("Dispatch_Id", 0)))
+        if not rows:
+            raise SystemExit(f"no {counter} rows for {sorted(set(kerns))} in {fn}")
+        return [(float(r["Counter_Value"]), r["Kernel_Name"]) for r in rows]
+    fr = per_dispatch(fetch_csv, "FETCH_SIZE")
+    wr = per_dispatch(write_csv, "WRITE_SIZE")
+    src = (f"{os.path.basename(os.path.dirname(fetch_csv))}, "
+           f"{os.path.basename(os.path.dirname(write_csv))}")
+    if sweep:
+        if len(fr) != len(tuns) or len(wr) != len(tuns):
+            raise SystemExit(f"{len(fr)}/{len(wr)} product dispatches for {len(tuns)} launch choices")
+        groups = [([f], [w]) for f, w in zip(fr, wr)]
+    else:
+        groups = [(fr, wr)]
     try:
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
     except (OSError, ValueError):
         tr = {}
-    tr[f"{workload}/{path}/{precision}/{tuning_key(tun)}"] = ent
+    for t, kern, (fs, ws) in zip(tuns, kerns, groups):
+        if any(kern not in n for _, n in fs + ws):
+            raise SystemExit(f"dispatch of another kernel where {kern} ({tuning_key(t)}) ran")
+        fk = sum(v for v, _ in fs) / len(fs)
+        wk = sum(v for v, _ in ws) / len(ws)
+        ent = dict(kernel=kern, fetch_kib=fk, write_kib=wk, dispatches=[len(fs), len(ws)],
+                   bytes_per_dispatch=round(1024.0 * (fk * FETCH_SCALE + wk * WRITE_SCALE)),
+                   problems_per_dispatch=problems, source=src)
+        tr[f"{workload}/{path}/{precision}/{tuning_key(t)}"] = ent
+        print(tuning_key(t), json.dumps(ent))
     with open(TRAFFIC_FILE, "w") as f:
         json.dump(tr, f, indent=1, sort_keys=True)
-    print(json.dumps(ent))
 
 
 def host_cpu():
@@ -502,6 +564,11 @@ def main():
                     help="record the product kernel's per-dispatch HBM traffic from two "
                          "rocprofv3 --pmc counter_collection.csv files in profiles/traffic.json")
     ap.add_argument("--problems-per-dispatch", type=int, default=0)
+    ap.add_argument("--product-sweep", default=None, metavar="SPEC",
+                    help="run one analysis per launch choice ('tuner': every product-side "
+                         "choice the autotuner can reach; or order,bucket_bits,xcd,tile;...) "
+                         "for per-dispatch counters; with --summarize-pmc, the same list "
+                         "names the dispatches")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch choices instead of letkf_gpu_autotune")
     ap.add_argument("--tuning", default=None,
@@ -525,7 +592,10 @@ def main():
     args.tun_fixed = tun_fixed
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, args.config, args.path, args.precision,
-                      args.problems_per_dispatch, tun_fixed)
+                      args.problems_per_dispatch, tun_fixed, args.product_sweep)
+        return
+    if args.product_sweep:
+        product_sweep(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; a multi-GPU "

@@ -354,7 +354,8 @@ __global__ __launch_bounds__(256) void k_finalize(LocDev L, StateDev S, VarsDev 
                                                   float rtpp, float mul,
                                                   double* __restrict__ ana,
                                                   double* __restrict__ amean,
-                                                  double* __restrict__ asprd) {
+                                                  double* __restrict__ asprd,
+                                                  bool skip_solved) {
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t t = (int64_t)blockIdx.x * 4 + wid;
   if (t >= C.ngp) return;
@@ -383,11 +384,14 @@ __global__ __launch_bounds__(256) void k_finalize(LocDev L, StateDev S, VarsDev 
     double x0, x1;      // transformed perturbation, or background member
     float b0, b1;       // background member
     float bm, bs;       // background mean and spread
+    bool skip;          // post-processed elsewhere
   };
   auto fetch = [&](int64_t s) {
     SlabIn r;
     const int g = slab_group(L, V, (int)s, mld);
     const bool solved = g >= 0 && C.solve[t * C.G + g];
+    r.skip = skip_solved && solved;
+    if (r.skip) return r;
     r.b0 = m0 ? xb0[k * s + lane] : 0.0f;
     r.b1 = m1 ? xb0[k * s + lane + 64] : 0.0f;
     r.x0 = solved ? (m0 ? xa0[k * s + lane] : 0.0) : (double)r.b0;
@@ -402,6 +406,7 @@ __global__ __launch_bounds__(256) void k_finalize(LocDev L, StateDev S, VarsDev 
     const SlabIn in = cur;
     cur = nxt;
     if (s + 2 < ns) nxt = fetch(s + 2);
+    if (in.skip) continue;
     const double xb_0 = (double)in.b0;
     const double xb_1 = (double)in.b1;
     double x0 = in.x0;
@@ -459,12 +464,10 @@ __global__ __launch_bounds__(256) void k_finalize(LocDev L, StateDev S, VarsDev 
 
 void launch_finalize(hipStream_t st, const LocDev& L, const StateDev& S,
                      const VarsDev& V, const ChunkDev& C, const DiagDev& D,
-                     float rtps, float rtpp, float mul, const double* ana_in,
-                     double* ana, double* amean, double* asprd) {
-  (void)ana_in;
+                     const FinArgs& F, double* ana, bool skip_solved) {
   if (C.ngp <= 0) return;
-  k_finalize<<<(unsigned)((C.ngp + 3) / 4), 256, 0, st>>>(L, S, V, C, D, rtps, rtpp,
-                                                          mul, ana, amean, asprd);
+  k_finalize<<<(unsigned)((C.ngp + 3) / 4), 256, 0, st>>>(
+      L, S, V, C, D, F.rtps, F.rtpp, F.mul, ana, F.amean, F.asprd, skip_solved);
 }
 
 }  // namespace lk

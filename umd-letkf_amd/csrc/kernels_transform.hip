@@ -7,7 +7,8 @@
 //  k_apply_mat     one 256-thread block per problem; wave w computes members
 //                  32w..32w+31 of x_a = T x_b for 32 slabs at a time on
 //                  MFMA (32x32x2 FP32, or 32x32x16 BF16 / bf16x3 with FP32
-//                  accumulation) and writes the FP64 analysis;
+//                  accumulation), post-processes them (FIN) and writes the
+//                  FP64 analysis;
 //  k_apply_mat_f64 the same product in FP64 on the vector units (the plain
 //                  path).
 #include <hip/hip_runtime.h>
@@ -147,14 +148,141 @@ __global__ __launch_bounds__(128) void k_mat_rational(int k, ChunkDev C, double*
   else mat_rational_rows<kMaxNodes>(k, c, rfirst, nn, m, wn, Us, Gs, Bp);
 }
 
+// sum over a slab of the tile: the lane's registers (summed by the caller),
+// the two halves of the wave, then the four waves through LDS (every wave
+// reads the four partial sums in the same order)
+__device__ __forceinline__ double tile_slab_sum(double p, double (*red)[32], int wv, int i,
+                                                int half) {
+  p += __shfl_xor(p, 32);
+  if (half == 0) red[wv][i] = p;
+  __syncthreads();
+  return (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]);
+}
+
+// post-processing of one slab per lane pair (i, i + 32) of every wave, from
+// the product's registers (register r: member 32 wv + 8 (r/4) + 4 half + r%4,
+// valid where bit r of mvalid is set); the same steps and arithmetic as
+// k_finalize.  Every wave of the block calls it (it holds barriers).
+__device__ __forceinline__ void apply_finalize(const f32x16& acc, unsigned mvalid, bool slab_ok,
+                                               const int* sl, int qi, int k, int64_t ij,
+                                               int64_t ns, float bmf, float bsf, int v,
+                                               const VarsDev& V, const FinArgs& F,
+                                               const float* xbrow, double (*red)[4][32], int wv,
+                                               int i, int half, double* xa0) {
+  const double dk = (double)k, dk1 = (double)(k - 1);
+  const double bm = (double)bmf, bs = (double)bsf;
+  double x[16];
+  auto msum = [&]() {
+    double p = 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) p += ((mvalid >> r) & 1u) ? x[r] : 0.0;
+    return p;
+  };
+  auto msq = [&]() {
+    double p = 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) p += ((mvalid >> r) & 1u) ? x[r] * x[r] : 0.0;
+    return p;
+  };
+  // recenter (src/letkf_solver.F90:299-303)
+#pragma unroll
+  for (int r = 0; r < 16; ++r) x[r] = (double)acc[r] + bm;
+  double mean = tile_slab_sum(msum(), red[0], wv, i, half) / dk;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) x[r] -= mean;
+  // increment clipping (:308-318)
+  double blo = 0.0, bhi = 0.0;
+  if (v >= 0) {
+    const double inc = (double)V.inc_max[v];
+    blo = (double)V.bounds[2 * v];
+    bhi = (double)V.bounds[2 * v + 1];
+    double r = mean - bm;
+    r = fmax(r, -inc);
+    r = fmin(r, inc);
+    r = fmax(r, blo - bm);
+    r = fmin(r, bhi - bm);
+    mean = r + bm;
+  }
+  // RTPS (:322-335)
+  if (F.rtps > 0.0f) {
+    const double sa = sqrt(tile_slab_sum(msq(), red[1], wv, i, half) / dk1);
+    if (sa > 0.0) {
+      const double f = (double)F.rtps * (bs - sa) / sa + 1.0;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) x[r] *= f;
+    }
+  }
+  // RTPP (:339-343), the background members from the staged tile
+  if (F.rtpp > 0.0f) {
+    const double a = (double)F.rtpp;
+#pragma unroll
+    for (int r4 = 0; r4 < 4; ++r4) {
+      const float4 b = *reinterpret_cast<const float4*>(xbrow + 8 * r4);
+      x[4 * r4 + 0] = a * (double)b.x + (1.0 - a) * x[4 * r4 + 0];
+      x[4 * r4 + 1] = a * (double)b.y + (1.0 - a) * x[4 * r4 + 1];
+      x[4 * r4 + 2] = a * (double)b.z + (1.0 - a) * x[4 * r4 + 2];
+      x[4 * r4 + 3] = a * (double)b.w + (1.0 - a) * x[4 * r4 + 3];
+    }
+  }
+  // multiplicative (:347-349)
+  if (F.mul != 1.0f) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x[r] *= (double)F.mul;
+  }
+  // member bounds and re-mean (:357-371)
+  if (V.nvars > 0) {
+    if (v >= 0) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) x[r] = fmin(fmax(x[r], blo - mean), bhi - mean);
+    }
+    const double r = tile_slab_sum(v >= 0 ? msum() : 0.0, red[2], wv, i, half) / dk;
+    if (v >= 0) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) x[q] -= r;
+      mean += r;
+    }
+  }
+  // final spread and members (:375-379)
+  const double sp = sqrt(tile_slab_sum(msq(), red[3], wv, i, half) / dk1);
+  if (!slab_ok) return;
+  const int s = sl[qi];
+  double* op = xa0 + (int64_t)k * s;
+  const bool vec = (k & 3) == 0;
+#pragma unroll
+  for (int r4 = 0; r4 < 4; ++r4) {
+    const int mem = 32 * wv + 8 * r4 + 4 * half;
+    if (vec && mem + 4 <= k) {
+      double2* o2 = reinterpret_cast<double2*>(op + mem);
+      o2[0] = make_double2(x[4 * r4] + mean, x[4 * r4 + 1] + mean);
+      o2[1] = make_double2(x[4 * r4 + 2] + mean, x[4 * r4 + 3] + mean);
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (mem + u < k) op[mem + u] = x[4 * r4 + u] + mean;
+    }
+  }
+  if (wv == 0 && half == 0) {
+    F.amean[ns * ij + s] = mean;
+    F.asprd[ns * ij + s] = sp;
+  }
+}
+
 // x_a = T x_b for every slab of the group: A operand T (rows = members),
 // B operand x_b (columns = slabs), K = the ensemble index.  Lane (i, half)
 // holds K = 16 step + 8 half + e (e = 0..7) of row / column i in both
 // operands (the same K order on both sides, any order contracts the same).
-template <int PREC>
+//
+// FIN: the post-processing of src/letkf_solver.F90:299-379 follows the
+// product for the tile's slabs, in FP64 from the FP32 products, in
+// k_finalize's order (recenter, increment clip, RTPS, RTPP, multiplicative,
+// member bounds with re-mean, spread); its sums over a slab's members add
+// the 16 registers of a lane, the two halves of the wave and the four waves
+// (through LDS), and the background members for RTPP are read from the
+// staged tile.  The analysis is then written once, post-processed.
+template <int PREC, bool FIN>
 __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev V, ChunkDev C,
                                                    DiagDev D, const double* __restrict__ B,
-                                                   double* __restrict__ ana) {
+                                                   double* __restrict__ ana, FinArgs F) {
   constexpr int KS = 8;              // K steps of 16, k <= 128
   __shared__ int sl[2048];
   __shared__ int snsl;
@@ -240,6 +368,12 @@ __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev
       *reinterpret_cast<float4*>(&xs[buf][tid >> 3][4 * (tid & 7) + 32 * u]) = pre[u];
   };
   const int ntile = (nsl + 31) / 32;
+  // FIN: per-wave partial sums of the tile's slabs, one array per sum
+  __shared__ double red[FIN ? 4 : 1][4][32];
+  unsigned mvalid = 0;              // register r holds a member < k
+#pragma unroll
+  for (int r = 0; r < 16; ++r)
+    if (32 * wv + 8 * (r >> 2) + 4 * half + (r & 3) < k) mvalid |= 1u << r;
   fetch(0);
   stash(0);
   __syncthreads();
@@ -247,9 +381,19 @@ __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev
     const int cur = tl & 1;
     const int q0 = 32 * tl;
     if (tl + 1 < ntile) fetch(q0 + 32);
+    const int qi = q0 + i;
+    // FIN: the slab's background mean, spread and variable, loaded before
+    // the products
+    float bmf = 0.0f, bsf = 0.0f;
+    int var = -1;
+    if (FIN) {
+      const int s = sl[qi < nsl ? qi : 0];
+      bmf = S.mean[S.ns * ij + s];
+      bsf = S.sprd[S.ns * ij + s];
+      var = V.slab_var[s];
+    }
+    f32x16 acc = (f32x16){};
     if (rows) {
-      const int qi = q0 + i;
-      f32x16 acc = (f32x16){};
 #pragma unroll
       for (int st = 0; st < KS; ++st) {
         if (st < ks) {
@@ -272,6 +416,11 @@ __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev
           }
         }
       }
+    }
+    if (FIN) {
+      apply_finalize(acc, mvalid, qi < nsl, sl, qi, k, ij, S.ns, bmf, bsf, var, V, F,
+                     &xs[cur][i][32 * wv + 4 * half], red, wv, i, half, xa0);
+    } else if (rows) {
       // register r of lane (i, half): member 32 wv + 8 (r/4) + 4 half + r%4 of slab q0 + i
       if (qi < nsl) {
         double* op = xa0 + (int64_t)k * sl[qi];
@@ -371,10 +520,12 @@ __global__ __launch_bounds__(256) void k_apply_mat_f64(LocDev L, StateDev S, Var
 
 bool transform_mat_supported(int k) { return k >= 64 && k <= 128; }
 
+bool transform_mat_fuses_finalize(int prec) { return prec != LETKF_PREC_FP64; }
+
 void launch_transform_mat(hipStream_t st, int prec, const LocDev& L, const StateDev& S,
                           const VarsDev& V, const ChunkDev& C, const DiagDev& D, double* B,
                           const double* bvec, double* tau, double* dd, double* ee,
-                          const RatTab& R, double* ana, bool blk) {
+                          const RatTab& R, double* ana, bool blk, const FinArgs* fin) {
   const int64_t np = C.ngp * C.G;
   if (np <= 0) return;
   const int k = (int)S.k;
@@ -408,14 +559,19 @@ void launch_transform_mat(hipStream_t st, int prec, const LocDev& L, const State
   else if (nr <= 48) launch_mat_back_nr48(st, nb, k, C, B, bvec, tau);
   else if (nr <= 56) launch_mat_back_nr56(st, nb, k, C, B, bvec, tau);
   else launch_mat_back_nr64(st, nb, k, C, B, bvec, tau);
+  const FinArgs F = fin ? *fin : FinArgs{0.0f, 0.0f, 1.0f, nullptr, nullptr};
+  const bool f = fin != nullptr && transform_mat_fuses_finalize(prec);
   if (prec == LETKF_PREC_FP64)
     k_apply_mat_f64<<<nb, 256, 0, st>>>(L, S, V, C, D, B, ana);
   else if (prec == LETKF_PREC_FP32)
-    k_apply_mat<LETKF_PREC_FP32><<<nb, 256, 0, st>>>(L, S, V, C, D, B, ana);
+    (f ? k_apply_mat<LETKF_PREC_FP32, true> : k_apply_mat<LETKF_PREC_FP32, false>)
+        <<<nb, 256, 0, st>>>(L, S, V, C, D, B, ana, F);
   else if (prec == LETKF_PREC_BF16)
-    k_apply_mat<LETKF_PREC_BF16><<<nb, 256, 0, st>>>(L, S, V, C, D, B, ana);
+    (f ? k_apply_mat<LETKF_PREC_BF16, true> : k_apply_mat<LETKF_PREC_BF16, false>)
+        <<<nb, 256, 0, st>>>(L, S, V, C, D, B, ana, F);
   else
-    k_apply_mat<LETKF_PREC_BF16X3><<<nb, 256, 0, st>>>(L, S, V, C, D, B, ana);
+    (f ? k_apply_mat<LETKF_PREC_BF16X3, true> : k_apply_mat<LETKF_PREC_BF16X3, false>)
+        <<<nb, 256, 0, st>>>(L, S, V, C, D, B, ana, F);
 }
 
 }  // namespace lk

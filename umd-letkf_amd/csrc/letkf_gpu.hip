@@ -155,6 +155,7 @@ struct letkf_gpu_t {
   int tune_vec = 1;       // vector path when every group has few slabs
   int tune_fuse = 1;      // rational solve inside the two-problem tridiagonalization
   int tune_matblk = 1;    // 64 <= k <= 128: 2D-block reduction and back-transform
+  int tune_fusefin = 1;   // 64 <= k <= 128, mixed: post-processing inside the apply
   int tune_rat = 1;       // vector path: rational A^(-1/2), no eigensolve (1 batched
                           // path only, 2 the plain FP64 path too, 0 off)
   int tune_mat = 1;       // batched, 64 <= k <= 128: explicit T applied on MFMA
@@ -305,6 +306,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   if (const char* e = getenv("LETKF_GPU_TILE")) h->tune_tile = atoi(e) != 0;
   if (const char* e = getenv("LETKF_GPU_FUSE")) h->tune_fuse = atoi(e) != 0;
   if (const char* e = getenv("LETKF_GPU_MATBLK")) h->tune_matblk = atoi(e) != 0;
+  if (const char* e = getenv("LETKF_GPU_FUSEFIN")) h->tune_fusefin = atoi(e) != 0;
   if (const char* e = getenv("LETKF_GPU_BUCKET")) h->tune_bucket = std::min(3, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_BBITS")) h->tune_bbits = std::min(2, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_TRIREG")) h->tune_trireg = std::min(2, std::max(0, atoi(e)));
@@ -984,14 +986,26 @@ static RunViews run_views(const letkf_gpu_t* h) {
               h->d_ctr}};
 }
 
+static FinArgs fin_args(const letkf_gpu_t* h) {
+  return FinArgs{h->rtps, h->rtpp, h->mul, h->d_amean, h->d_asprd};
+}
+
+// the large-ensemble mixed path post-processes the slabs of solved problems
+// inside the apply; k_finalize then does the rest
+static bool fin_fused(const letkf_gpu_t* h) {
+  return h->use_mat && h->batched() && h->tune_fusefin &&
+         transform_mat_fuses_finalize(h->dims.precision);
+}
+
 // the transform of one chunk's problems and its post-processing, on st
 static hipError_t enqueue_solve(letkf_gpu_t* h, hipStream_t st, const RunViews& W,
                                 const ChunkDev& C) {
   if (h->use_mat) {
     const RatTab R{h->d_rat, h->d_ratn, kRatBuckets};
+    const FinArgs F = fin_args(h);
     launch_transform_mat(st, h->batched() ? h->dims.precision : LETKF_PREC_FP64, h->loc, W.S,
                          W.V, C, W.D, h->d_B, h->d_b, h->d_tau, h->d_dd, h->d_ee, R, h->d_ana,
-                         h->tune_matblk != 0);
+                         h->tune_matblk != 0, fin_fused(h) ? &F : nullptr);
   } else if (h->eigK) {
     RatTab R;
     if (h->use_rat) { R.tab = h->d_rat; R.n = h->d_ratn; R.nb = kRatBuckets; }
@@ -1162,8 +1176,7 @@ static int run_eager(letkf_gpu_t* h) {
     RUNCHK(hipEventRecord(ce[4], st));
     RUNCHK(enqueue_solve(h, st, W, C));
     RUNCHK(hipEventRecord(ce[5], st));
-    launch_finalize(st, h->loc, S, V, C, D, h->rtps, h->rtpp, h->mul, h->d_ana, h->d_ana,
-                    h->d_amean, h->d_asprd);
+    launch_finalize(st, h->loc, S, V, C, D, fin_args(h), h->d_ana, fin_fused(h));
     RUNCHK(hipGetLastError());
     RUNCHK(hipEventRecord(ce[6], st));
     slot_free[slot] = ce[6];
@@ -1242,8 +1255,7 @@ static hipError_t enqueue_analysis(letkf_gpu_t* h) {
     e = hipGetLastError();
     if (e == hipSuccess) e = enqueue_solve(h, st, W, C);
     if (e != hipSuccess) break;
-    launch_finalize(st, h->loc, W.S, W.V, C, W.D, h->rtps, h->rtpp, h->mul, h->d_ana, h->d_ana,
-                    h->d_amean, h->d_asprd);
+    launch_finalize(st, h->loc, W.S, W.V, C, W.D, fin_args(h), h->d_ana, fin_fused(h));
     e = hipGetLastError();
   }
   return e;

@@ -246,13 +246,24 @@ void launch_eigen_pipeline(hipStream_t st, const LocDev& L, const StateDev& S,
                            const int32_t* perm, bool fuse);
 // explicit transform matrix and its MFMA application (64 <= k <= 128)
 bool transform_mat_supported(int k);
+// post-processing parameters (src/letkf_solver.F90:299-379) and the
+// analysis mean and spread it writes
+struct FinArgs {
+  float rtps, rtpp, mul;
+  double* amean;
+  double* asprd;
+};
+// fin != nullptr (mixed precisions only): the apply post-processes the slabs
+// it transforms, and launch_finalize must then run with skip_solved
+bool transform_mat_fuses_finalize(int prec);
 void launch_transform_mat(hipStream_t st, int prec, const LocDev& L, const StateDev& S,
                           const VarsDev& V, const ChunkDev& C, const DiagDev& D, double* B,
                           const double* bvec, double* tau, double* dd, double* ee,
-                          const RatTab& R, double* ana, bool blk);
+                          const RatTab& R, double* ana, bool blk, const FinArgs* fin);
+// skip_solved: only masked gridpoints and the slabs of groups without a
+// solved problem (the rest were post-processed by the fused apply)
 void launch_finalize(hipStream_t st, const LocDev& L, const StateDev& S,
                      const VarsDev& V, const ChunkDev& C, const DiagDev& D,
-                     float rtps, float rtpp, float mul, const double* ana_in,
-                     double* ana, double* amean, double* asprd);
+                     const FinArgs& F, double* ana, bool skip_solved);
 
 }  // namespace lk
