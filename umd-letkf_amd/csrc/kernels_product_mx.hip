@@ -25,10 +25,11 @@
 // Accumulation.  The MFMAs accumulate in FP32.  For PREC_FP32 and
 // PREC_BF16X4 the accumulators are compensated: every F observations (the
 // fold period, a power of two >= 32) the partial sums are added into FP32
-// running sums S by the error-free TwoSum (S + a = s + e exactly), the
-// rounding errors e collected in a per-wave carry array in LDS, and the
-// partial sums restart from zero.  The result S + carry + partial is formed
-// in FP64.  The error of an m-observation product then grows like
+// running sums S by the error-free TwoSum (S + a = s + e exactly), and the
+// partial sums restart from the rounding error e (Kahan's compensation
+// carried forward: e, about 2^-24 |S|, joins the next block's sum, whose own
+// rounding is of the same order).  The result S + partial is formed in
+// FP64.  The error of an m-observation product then grows like
 // F sqrt(m) rounding units instead of m^1.5 / 2 (products of FP32 operands
 // summed exactly would be ~1e-9 from the FP64 analysis; the FP32 partial
 // sums are what is left).  PREC_BF16X3 and PREC_BF16 keep plain FP32
@@ -63,6 +64,7 @@ __device__ __forceinline__ int obs_slot(int half, int e) {
 // (0, 0): column 0 with weight 0.  The padded columns carry the departure in
 // row k and zeros above it, so no row or range test is needed.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // column byte offsets of the lane's 8 observations of a step
 struct ListRegs {
@@ -176,23 +178,29 @@ __device__ __forceinline__ void mx_step(f32x16* acc, const MxFrag<NB>& in) {
   }
 }
 
-// compensated accumulation: S += acc by TwoSum, the rounding error into the
-// wave's carry array (row r of block b of lane l at carry[(16 b + r) * 64 + l]),
-// acc restarts at zero
+// compensated accumulation: S += acc by TwoSum, acc restarts from the
+// rounding error; the running sums of block b, four at a time, are the
+// lane's own LDS slots Sl[64 (4 b + q)], out of the registers
 template <int NBLK>
-__device__ __forceinline__ void mx_fold(f32x16* acc, f32x16* S, float* carry, int lane) {
+__device__ __forceinline__ void mx_fold(f32x16* acc, f32x4* Sl) {
 #pragma clang fp contract(off)
 #pragma unroll
   for (int b = 0; b < NBLK; ++b)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const float a = acc[b][r], x = S[b][r];
-      const float t = x + a;
-      const float bb = t - x;
-      const float e = (x - (t - bb)) + (a - bb);
-      S[b][r] = t;
-      carry[(16 * b + r) * 64 + lane] += e;
-      acc[b][r] = 0.0f;
+    for (int q = 0; q < 4; ++q) {
+      const f32x4 x4 = Sl[64 * (4 * b + q)];
+      f32x4 t4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = 4 * q + u;
+        const float a = acc[b][r], x = x4[u];
+        const float t = x + a;
+        const float bb = t - x;
+        const float e = (x - (t - bb)) + (a - bb);
+        t4[u] = t;
+        acc[b][r] = e;
+      }
+      Sl[64 * (4 * b + q)] = t4;
     }
 }
 
@@ -222,14 +230,11 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
   f32x16 acc[NBLK];
 #pragma unroll
   for (int b = 0; b < NBLK; ++b) acc[b] = (f32x16){};
-  // compensated accumulation: running sums and the wave's carry array
-  f32x16 S[COMP ? NBLK : 1];
-  __shared__ float carry[COMP ? NBLK * 16 * 64 : 1];
+  // compensated accumulation: running sums (LDS, the lane's own slots)
+  __shared__ f32x4 Ssh[COMP ? NBLK * 4 * 64 : 1];
   if (COMP) {
 #pragma unroll
-    for (int b = 0; b < NBLK; ++b) S[b] = (f32x16){};
-#pragma unroll
-    for (int i = 0; i < NBLK * 16; ++i) carry[i * 64 + lane] = 0.0f;
+    for (int q = 0; q < NBLK * 4; ++q) Ssh[64 * q + lane] = (f32x4){};
   }
 
   // software pipeline: the list entries of a step are loaded two steps
@@ -258,7 +263,7 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
     mx_step<NB, PREC>(acc, fb);
     __builtin_amdgcn_sched_barrier(0);
     // every fold_mask + 1 observations (wave-uniform)
-    if (COMP && ((j0 + 32) & fold_mask) == 0) mx_fold<NBLK>(acc, S, carry, lane);
+    if (COMP && ((j0 + 32) & fold_mask) == 0) mx_fold<NBLK>(acc, Ssh + lane);
   }
 
   // 32x32 f32 accumulator: register r of lane l holds row 8(r/4) + 4(l/32) + r%4
@@ -275,7 +280,7 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
         const int row = 32 * rb + 8 * (r >> 2) + 4 * half + (r & 3);
         const int col = 32 * cb + i;
         double v = (double)acc[b][r];
-        if (COMP) v = (double)S[b][r] + (v + (double)carry[(16 * b + r) * 64 + lane]);
+        if (COMP) v += (double)Ssh[64 * (4 * b + (r >> 2)) + lane][r & 3];
         if (row < k && col < k) {
           Bp[row + (int64_t)k * col] = v;
           Bp[col + (int64_t)k * row] = v;
@@ -300,7 +305,6 @@ __global__ __launch_bounds__(64) void k_product_mx(ObsDev O, int k, ChunkDev C,
 // precisions and compensated accumulation as above (PREC_FP32 keeps the
 // 32x32 kernel).
 // ---------------------------------------------------------------------------
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <int NT>
 struct T16Frag {
@@ -388,22 +392,27 @@ __device__ __forceinline__ void t16_mfma(f32x4* acc, const bf16x8* hi, const bf1
       acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[rt], hi[ct], acc[b], 0, 0, 0);
 }
 
-// compensated accumulation on 16x16 tiles (4 values per lane per tile)
+// compensated accumulation on 16x16 tiles (4 values per lane per tile); the
+// running sums of tile b are the lane's own LDS slot Sl[64 b], which keeps
+// them out of the registers (two waves per SIMD become three at NT = 4)
 template <int NTILE>
-__device__ __forceinline__ void t16_fold(f32x4* acc, f32x4* S, float* carry, int lane) {
+__device__ __forceinline__ void t16_fold(f32x4* acc, f32x4* Sl) {
 #pragma clang fp contract(off)
 #pragma unroll
-  for (int b = 0; b < NTILE; ++b)
+  for (int b = 0; b < NTILE; ++b) {
+    const f32x4 x4 = Sl[64 * b];
+    f32x4 t4;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float a = acc[b][r], x = S[b][r];
+      const float a = acc[b][r], x = x4[r];
       const float t = x + a;
       const float bb = t - x;
       const float e = (x - (t - bb)) + (a - bb);
-      S[b][r] = t;
-      carry[(4 * b + r) * 64 + lane] += e;
-      acc[b][r] = 0.0f;
+      t4[r] = t;
+      acc[b][r] = e;
     }
+    Sl[64 * b] = t4;
+  }
 }
 
 template <int NT, int PREC>
@@ -429,13 +438,10 @@ __global__ __launch_bounds__(64) void k_product_t16(ObsDev O, int k, ChunkDev C,
   f32x4 acc[NTILE];
 #pragma unroll
   for (int b = 0; b < NTILE; ++b) acc[b] = (f32x4){};
-  f32x4 S[COMP ? NTILE : 1];
-  __shared__ float carry[COMP ? NTILE * 4 * 64 : 1];
+  __shared__ f32x4 Ssh[COMP ? NTILE * 64 : 1];
   if (COMP) {
 #pragma unroll
-    for (int b = 0; b < NTILE; ++b) S[b] = (f32x4){};
-#pragma unroll
-    for (int i = 0; i < NTILE * 4; ++i) carry[i * 64 + lane] = 0.0f;
+    for (int b = 0; b < NTILE; ++b) Ssh[64 * b + lane] = (f32x4){};
   }
 
   // f: the columns of the step in flight; la, lb: lists two and one steps
@@ -456,7 +462,7 @@ __global__ __launch_bounds__(64) void k_product_t16(ObsDev O, int k, ChunkDev C,
     t16_list(la, ir, j0 + 96, lane);
     __builtin_amdgcn_sched_barrier(0);
     t16_mfma<NT, PREC>(acc, hi, lo);
-    if (COMP && ((j0 + 32) & fold_mask) == 0) t16_fold<NTILE>(acc, S, carry, lane);
+    if (COMP && ((j0 + 32) & fold_mask) == 0) t16_fold<NTILE>(acc, Ssh + lane);
   }
 
   // 16x16 f32 accumulator: register r of lane l holds row 4 (l / 16) + r,
@@ -469,12 +475,13 @@ __global__ __launch_bounds__(64) void k_product_t16(ObsDev O, int k, ChunkDev C,
   for (int rt = 0; rt < NT; ++rt)
 #pragma unroll
     for (int ct = rt; ct < NT; ++ct, ++b) {
+      const f32x4 sb = COMP ? Ssh[64 * b + lane] : (f32x4){};
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = 16 * rt + 4 * g + r;
         const int col = 16 * ct + i;
         double v = (double)acc[b][r];
-        if (COMP) v = (double)S[b][r] + (v + (double)carry[(4 * b + r) * 64 + lane]);
+        if (COMP) v += (double)sb[r];
         if (row < k && col < k) {
           Bp[row + (int64_t)k * col] = v;
           Bp[col + (int64_t)k * row] = v;
