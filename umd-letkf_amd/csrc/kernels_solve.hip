@@ -366,17 +366,43 @@ __global__ __launch_bounds__(256) void k_finalize(LocDev L, StateDev S, VarsDev 
   double* xa0 = ana + (int64_t)k * ns * ij;
   const bool m0 = lane < k, m1 = lane + 64 < k;
   if (S.mask[ij]) {
-    for (int64_t s = 0; s < ns; ++s) {
-      if (m0) xa0[k * s + lane] = (double)xb0[k * s + lane];
-      if (m1) xa0[k * s + lane + 64] = (double)xb0[k * s + lane + 64];
-      if (lane == 0) {
-        amean[ns * ij + s] = (double)S.mean[ns * ij + s];
-        asprd[ns * ij + s] = (double)S.sprd[ns * ij + s];
+    // the background unchanged: the gridpoint's k x ns block is contiguous,
+    // copied 512 elements per round with 8 loads of a lane in flight
+    const int64_t n = (int64_t)k * ns;
+    for (int64_t e0 = 0; e0 < n; e0 += 512) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t e = e0 + 64 * u + lane;
+        v[u] = e < n ? xb0[e] : 0.0f;
       }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t e = e0 + 64 * u + lane;
+        if (e < n) xa0[e] = (double)v[u];
+      }
+    }
+    for (int64_t s = lane; s < ns; s += 64) {
+      amean[ns * ij + s] = (double)S.mean[ns * ij + s];
+      asprd[ns * ij + s] = (double)S.sprd[ns * ij + s];
     }
     return;
   }
   const int mld = D.mld[ij];
+  if (skip_solved) {
+    // nothing to do where every slab's group was solved (and post-processed
+    // by the apply): the slabs' flags are checked 64 at a time
+    bool any = false;
+    for (int64_t s0 = 0; s0 < ns && !any; s0 += 64) {
+      bool need = false;
+      if (s0 + lane < ns) {
+        const int g = slab_group(L, V, (int)(s0 + lane), mld);
+        need = !(g >= 0 && C.solve[t * C.G + g]);
+      }
+      any = __ballot(need) != 0;
+    }
+    if (!any) return;
+  }
   const double dk = (double)k, dk1 = (double)(k - 1);
   // a slab's inputs are loaded two slabs ahead of its post-processing, so
   // that the loads of later slabs are in flight during the wave sums
