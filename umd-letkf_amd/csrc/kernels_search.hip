@@ -39,18 +39,35 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// exclusive prefix sum over the wave's lanes, and the total
+__device__ __forceinline__ int wave_excl_scan(int v, int lane, int* total) {
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  *total = __shfl(x, 63, 64);
+  return x - v;
+}
 __device__ __forceinline__ int lane_rank(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// the localizer's scalars that the weights use (copied out of the kernel
+// arguments, so that no reference to them is held)
+struct LocScalars {
+  float tloc_prof, tloc_sat, vt_scale;
+};
+
 // localization weight of one kept-candidate obs for group g (FP32 result)
-__device__ __forceinline__ float loc_weight(const LocDev& L, int g, float hz,
+__device__ __forceinline__ float loc_weight(int cls, const LocScalars& L, int g, float hz,
                                             float d, float Lp, float Ls,
                                             uint8_t surf, float twp, float tws,
                                             float zdim, const float* lev_depth) {
 #pragma clang fp contract(off)
-  if (L.cls == LETKF_LOC_OCEAN) {
+  if (cls == LETKF_LOC_OCEAN) {
     if (surf) {
       if (g == 1) return 0.0f;
       float r = dm::gc(d, Ls);
@@ -61,7 +78,7 @@ __device__ __forceinline__ float loc_weight(const LocDev& L, int g, float hz,
     if (L.tloc_prof > 0.0f) r = r * twp;
     return r;
   }
-  if (L.cls == LETKF_LOC_VGAUSS) {
+  if (cls == LETKF_LOC_VGAUSS) {
     const double dz = (double)zdim - (double)lev_depth[g];
     const double sc = (double)L.vt_scale;
     const double v = dm::exp_neg(-(dz * dz) / (2.0 * sc * sc));
@@ -70,10 +87,32 @@ __device__ __forceinline__ float loc_weight(const LocDev& L, int g, float hz,
   return hz;
 }
 
-__device__ __forceinline__ bool group_has_slabs(const LocDev& L, int g, int mld) {
-  if (L.cls == LETKF_LOC_OCEAN && L.vtloc_surf == 1) return g == 0 || L.max_var_lev > mld;
-  if (L.cls == LETKF_LOC_VGAUSS) return g < L.max_var_lev;
+__device__ __forceinline__ bool group_has_slabs(int cls, const LocDev& L, int g, int mld) {
+  if (cls == LETKF_LOC_OCEAN && L.vtloc_surf == 1) return g == 0 || L.max_var_lev > mld;
+  if (cls == LETKF_LOC_VGAUSS) return g < L.max_var_lev;
   return true;
+}
+
+// dm::interp on a table held in the kernel arguments, with constant indices
+// only (a table read at a computed index would make the compiler copy the
+// whole argument block to private memory); the same arithmetic
+__device__ __forceinline__ float interp_arg(int n, const float (&lat)[kTab],
+                                            const float (&dist)[kTab], float x) {
+#pragma clang fp contract(off)
+  const float a = fabsf(x);
+  int i = n - 1;
+#pragma unroll
+  for (int j = kTab - 1; j >= 0; --j)
+    if (j < n - 1 && !(a > lat[j])) i = j;
+  if (i == 0) return dist[0];
+  float l1 = 0.0f, d1 = 0.0f, l0 = 0.0f, d0 = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kTab; ++j) {
+    if (j == i) { l1 = lat[j]; d1 = dist[j]; }
+    if (j == i - 1) { l0 = lat[j]; d0 = dist[j]; }
+  }
+  const float d = (a - l0) / (l1 - l0);
+  return d * d1 + (1.0f - d) * d0;
 }
 
 // Search radius of a gridpoint (src/letkf_solver.F90:210-212) and the
@@ -82,14 +121,14 @@ struct Radii {
   float Lh, Lp, Ls, maxhz, rs;
 };
 
-__device__ __forceinline__ Radii search_radii(const LocDev& L, float lat_ij) {
+__device__ __forceinline__ Radii search_radii(int cls, const LocDev& L, float lat_ij) {
   Radii R{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  if (L.cls == LETKF_LOC_OCEAN) {
-    R.Lp = dm::interp(L.n_prof, L.prof_lat, L.prof_dist, lat_ij);
-    R.Ls = dm::interp(L.n_sat, L.sat_lat, L.sat_dist, lat_ij);
+  if (cls == LETKF_LOC_OCEAN) {
+    R.Lp = interp_arg(L.n_prof, L.prof_lat, L.prof_dist, lat_ij);
+    R.Ls = interp_arg(L.n_sat, L.sat_lat, L.sat_dist, lat_ij);
     R.maxhz = R.Lp > R.Ls ? R.Lp : R.Ls;
   } else {
-    R.Lh = dm::interp(L.n_hz, L.hz_lat, L.hz_dist, lat_ij);
+    R.Lh = interp_arg(L.n_hz, L.hz_lat, L.hz_dist, lat_ij);
     R.maxhz = R.Lh;
   }
   {
@@ -188,7 +227,7 @@ __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_wave_barrier();
 }
 
-template <int PASS>
+template <int PASS, int CLS>
 __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
                                                 VarsDev V, ChunkDev C, DiagDev D) {
   constexpr bool FILL = PASS == kFill || PASS == kFillDiag;
@@ -197,9 +236,12 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
   __shared__ double s_sum[4][kMaxGroups];
   __shared__ uint64_t s_hash[4][kMaxGroups];
   __shared__ int64_t s_lim[4][kMaxGroups];       // fill: end of each list's space
-  __shared__ int64_t s_qs[4][kQueue];             // queued candidates: obs index
-  __shared__ double s_qc[4][kQueue];              //   and squared chord
-  __shared__ double s_dl[4][64];                  // longitude half-width per band
+  __shared__ int32_t s_qs[4][kQueue];             // queued candidates: obs index,
+  __shared__ double s_qc[4][kQueue];              //   squared chord,
+  __shared__ float s_qe[4][kQueue];               //   error variance
+  __shared__ int32_t s_qo[4][kQueue];             //   and original index
+  __shared__ int32_t s_rsb[4][129];               // cover ranges: first obs
+  __shared__ int32_t s_roff[4][130];              //   and offset in the candidate stream
   const int wid = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t t = (int64_t)blockIdx.x * 4 + wid;
@@ -233,12 +275,13 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
   }
 
   const float lat_ij = S.lat[ij];
-  const Radii R = search_radii(L, lat_ij);
+  const Radii R = search_radii(CLS, L, lat_ij);
+  const LocScalars LS{L.tloc_prof, L.tloc_sat, L.vt_scale};
   const float Lh = R.Lh, Lp = R.Lp, Ls = R.Ls, maxhz = R.maxhz, rs = R.rs;
   const double thr2 = dm::thr2(rs);
 
   int mld = 0;                                        // calc_mldlvl_bkgt
-  if (L.cls == LETKF_LOC_OCEAN && L.vtloc_surf == 1) {
+  if (CLS == LETKF_LOC_OCEAN && L.vtloc_surf == 1) {
     const float* mean = S.mean + S.ns * ij;
     const float t0 = mean[L.bkg_t_var_idx];
     mld = L.bkg_t_var_len;
@@ -268,15 +311,19 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     fd_cur = C.off[t];
     fd_lim = fd_cur + C.cnt[t];
   }
-  auto localize = [&](int cnt) {
+  auto localize = [&](int cnt) __attribute__((always_inline)) {
     wave_lds_fence();
     const bool inside = lane < cnt;
     int64_t s = 0;
     double c2 = 9.0;
+    float oerr = 1.0f;
+    int32_t oid = 0;
     if (inside) {
       const int e = (qh + lane) & (kQueue - 1);
       s = s_qs[wid][e];
       c2 = s_qc[wid][e];
+      oerr = s_qe[wid][e];
+      oid = s_qo[wid][e];
     }
     wave_lds_fence();
     float d = 0.0f, hz = 0.0f, zd = 0.0f, twp = 1.0f, tws = 1.0f;
@@ -284,14 +331,14 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     uint64_t mix = 0;
     if (inside) {
       d = dm::dist(c2);
-      if (L.cls != LETKF_LOC_OCEAN) hz = dm::gc(d, Lh);
-      if (L.cls == LETKF_LOC_OCEAN) { sf = O.surf[s]; twp = O.tw_prof[s]; tws = O.tw_sat[s]; }
-      if (L.cls == LETKF_LOC_VGAUSS) zd = O.zdim[s];
-      if (STATS) mix = dm::mix64((uint64_t)O.oidx[s]);
+      if (CLS != LETKF_LOC_OCEAN) hz = dm::gc(d, Lh);
+      if (CLS == LETKF_LOC_OCEAN) { sf = O.surf[s]; twp = O.tw_prof[s]; tws = O.tw_sat[s]; }
+      if (CLS == LETKF_LOC_VGAUSS) zd = O.zdim[s];
+      if (STATS) mix = dm::mix64((uint64_t)oid);
     }
     for (int g = 0; g < G; ++g) {
       float w = 0.0f;
-      if (inside) w = loc_weight(L, g, hz, d, Lp, Ls, sf, twp, tws, zd, V.lev_depth);
+      if (inside) w = loc_weight(CLS, LS, g, hz, d, Lp, Ls, sf, twp, tws, zd, V.lev_depth);
       const bool keep = inside && (w > 1.0e-4f);   // src/letkf_solver.F90:251
       const uint64_t kmask = __ballot(keep);
       if (kmask == 0) continue;
@@ -324,7 +371,7 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
             // (1 ulp each; the products round far more)
             C.list_idx[pos] = C.list_mode ? (int32_t)(s * O.kp * 4) : (int32_t)s;
             C.list_w[pos] = C.list_mode
-                                ? __builtin_amdgcn_sqrtf(w * __builtin_amdgcn_rcpf(O.err[s]))
+                                ? __builtin_amdgcn_sqrtf(w * __builtin_amdgcn_rcpf(oerr))
                                 : w;
           }
         }
@@ -336,44 +383,114 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     qn -= cnt;
   };
 
-  // lane i bounds band b0 + i; bands past the 64th (caps wider than 32
-  // degrees) take whole longitude circles
-  s_dl[wid][lane] = b0 + lane <= b1 ? cover_halfwidth(cv, b0 + lane) : 180.0;
-  wave_lds_fence();
-  for (int b = b0; b <= b1; ++b) {
-    const double dl = b - b0 < 64 ? s_dl[wid][b - b0] : 180.0;
-    if (dl < 0.0) continue;
-    int rc0[2], rc1[2];
-    const int nr = cover_ranges(cv, dl, rc0, rc1);
-    for (int r = 0; r < nr; ++r) {
-      const int64_t sb = O.cell_start[b * kLonCells + rc0[r]];
-      const int64_t se = O.cell_start[b * kLonCells + rc1[r] + 1];
-      if (PASS != kFill) ncand += se - sb;
-      // each batch's positions are loaded while the previous one is tested
-      double px = 0.0, py = 0.0, pz = 0.0;
-      if (sb + lane < se) { px = O.x[sb + lane]; py = O.y[sb + lane]; pz = O.z[sb + lane]; }
-      for (int64_t base = sb; base < se; base += 64) {
-        const int64_t s = base + lane;
-        const bool valid = s < se;
-        const double cx = px, cy = py, cz = pz;
-        const int64_t sn = s + 64;
-        if (sn < se) { px = O.x[sn]; py = O.y[sn]; pz = O.z[sn]; }
-        double c2 = 9.0;
-        if (valid) c2 = dm::chord2(cx, cy, cz, gx, gy, gz);
-        const bool inside = c2 <= thr2;
-        const uint64_t imask = __ballot(inside);
-        if (imask == 0) continue;
-        nin += __popcll(imask);
-        // qn < 64 here, so the ring (128) has room for the whole batch
-        if (inside) {
-          const int e = (qh + qn + lane_rank(imask)) & (kQueue - 1);
-          s_qs[wid][e] = s;
-          s_qc[wid][e] = c2;
+  // The cover's cells: lane i finds the one or two longitude ranges of band
+  // b0 + i and their obs ranges; bands past the 64th of the cover take whole
+  // longitude circles, which follow one another in the cell order and make
+  // one more range.  The ranges, in band order, make one stream of
+  // candidates, scanned 64 at a time in the same order as band by band.  The
+  // positions (and the fields the localization needs) of a batch are loaded
+  // two batches ahead.
+  const bool need_err = FILL && C.list_mode;
+  struct Cand {
+    double x, y, z;
+    float e;
+    int32_t o, s;       // s < 0: past the stream
+  };
+  {
+    const int b = b0 + lane;
+    int nr = 0, sb0 = 0, len0 = 0, sb1 = 0, len1 = 0;
+    if (b <= b1) {
+      const double dl = cover_halfwidth(cv, b);
+      if (dl >= 0.0) {
+        int rc0[2], rc1[2];
+        nr = cover_ranges(cv, dl, rc0, rc1);
+        const int32_t* cs = O.cell_start + b * kLonCells;
+        sb0 = cs[rc0[0]];
+        len0 = cs[rc1[0] + 1] - sb0;
+        if (nr == 2) {
+          sb1 = cs[rc0[1]];
+          len1 = cs[rc1[1] + 1] - sb1;
         }
-        qn += __popcll(imask);
-        if (qn >= 64) localize(64);
       }
     }
+    int nrt = 0, T = 0;
+    const int slot = wave_excl_scan(nr, lane, &nrt);
+    const int off = wave_excl_scan(len0 + len1, lane, &T);
+    if (nr >= 1) {
+      s_rsb[wid][slot] = sb0;
+      s_roff[wid][slot] = off;
+    }
+    if (nr == 2) {
+      s_rsb[wid][slot + 1] = sb1;
+      s_roff[wid][slot + 1] = off + len0;
+    }
+    if (b1 - b0 >= 64) {
+      const int st0 = O.cell_start[(b0 + 64) * kLonCells];
+      const int st1 = O.cell_start[(b1 + 1) * kLonCells];
+      if (lane == 0) {
+        s_rsb[wid][nrt] = st0;
+        s_roff[wid][nrt] = T;
+      }
+      nrt += 1;
+      T += st1 - st0;
+    }
+    if (lane == 0) s_roff[wid][nrt] = T;
+    wave_lds_fence();
+    if (PASS != kFill) ncand += T;
+    int rl = 0;                       // range of this lane's next load
+    // the loads are issued by every lane (past the stream: observation 0),
+    // so that the wait for a batch counts the loads issued after it
+    auto fetch = [&](Cand& c, int q) __attribute__((always_inline)) {
+      const int p = q + lane;
+      int so = 0;
+      if (p < T) {
+        while (p >= s_roff[wid][rl + 1]) ++rl;
+        so = s_rsb[wid][rl] + (p - s_roff[wid][rl]);
+      }
+      c.s = p < T ? so : -1;
+      c.x = O.x[so];
+      c.y = O.y[so];
+      c.z = O.z[so];
+      if (FILL) c.e = O.err[so];
+      if (STATS) c.o = O.oidx[so];
+    };
+    auto scan = [&](const Cand& c) __attribute__((always_inline)) {
+      double c2 = 9.0;
+      if (c.s >= 0) c2 = dm::chord2(c.x, c.y, c.z, gx, gy, gz);
+      const bool inside = c2 <= thr2;
+      const uint64_t imask = __ballot(inside);
+      if (imask == 0) return;
+      nin += __popcll(imask);
+      // qn < 64 here, so the ring (128) has room for the whole batch
+      if (inside) {
+        const int e = (qh + qn + lane_rank(imask)) & (kQueue - 1);
+        s_qs[wid][e] = c.s;
+        s_qc[wid][e] = c2;
+        s_qe[wid][e] = need_err ? c.e : 1.0f;
+        s_qo[wid][e] = STATS ? c.o : 0;
+      }
+      qn += __popcll(imask);
+      if (qn >= 64) localize(64);
+    };
+    // three batches in registers, each in the same ones throughout (the
+    // loop is unrolled three times, so no batch is moved while its loads
+    // are in flight): the one scanned and the two loading
+    Cand ca, cb, cc;
+    if (T > 0) {
+      fetch(ca, 0);
+      fetch(cb, 64);
+    }
+    for (int q = 0; q < T; q += 192) {
+      fetch(cc, q + 128);
+      scan(ca);
+      if (q + 64 >= T) break;
+      fetch(ca, q + 192);
+      scan(cb);
+      if (q + 128 >= T) break;
+      fetch(cb, q + 256);
+      scan(cc);
+    }
+    wave_lds_fence();
   }
   if (qn > 0) localize(qn);
 
@@ -413,7 +530,7 @@ __global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
     unsigned long long nsolve = 0, npair = 0;
     for (int g = 0; g < G; ++g) {
       const int64_t m = PASS == kFillDiag ? s_cnt[wid][g] - C.off[t * G + g] : s_cnt[wid][g];
-      const bool sv = m > 0 && group_has_slabs(L, g, mld);
+      const bool sv = m > 0 && group_has_slabs(CLS, L, g, mld);
       nsolve += sv ? 1 : 0;
       npair += sv ? (unsigned long long)m : 0ull;
       C.cnt[t * G + g] = m;
@@ -455,7 +572,7 @@ __global__ __launch_bounds__(256) void k_count_cover(LocDev L, ObsDev O, StateDe
     return;
   }
   const float lat_ij = S.lat[ij];
-  const CapCover cv = cap_cover(search_radii(L, lat_ij).rs, lat_ij, S.lon[ij]);
+  const CapCover cv = cap_cover(search_radii(L.cls, L, lat_ij).rs, lat_ij, S.lon[ij]);
   int64_t ncand = 0;
   for (int b = cv.b0; b <= cv.b1; ++b) {
     const double dl = cover_halfwidth(cv, b);
@@ -501,6 +618,16 @@ void launch_list_guard(hipStream_t st, int64_t np, int64_t* cnt, const int64_t* 
   k_list_guard<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(np, cnt, off, solve, cap, ctr);
 }
 
+// the search kernel for localizer class CLS (its branches folded away)
+template <int CLS>
+static void launch_search_cls(hipStream_t st, int pass, unsigned nb, const LocDev& L,
+                              const ObsDev& O, const StateDev& S, const VarsDev& V,
+                              const ChunkDev& C, const DiagDev& D) {
+  if (pass == kFillDiag) k_search<kFillDiag, CLS><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+  else if (pass == kFill) k_search<kFill, CLS><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+  else k_search<kCountExact, CLS><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+}
+
 void launch_search(hipStream_t st, bool fill, const LocDev& L, const ObsDev& O,
                    const StateDev& S, const VarsDev& V, const ChunkDev& C,
                    const DiagDev& D) {
@@ -508,12 +635,15 @@ void launch_search(hipStream_t st, bool fill, const LocDev& L, const ObsDev& O,
   const unsigned nb = (unsigned)((C.ngp + 3) / 4);
   // one group: size the lists by the search set (chord test only) and count
   // the kept observations while writing them
-  if (C.G == 1) {
-    if (fill) k_search<kFillDiag><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
-    else k_count_cover<<<(unsigned)((C.ngp + 255) / 256), 256, 0, st>>>(L, O, S, C, D);
-  } else {
-    if (fill) k_search<kFill><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
-    else k_search<kCountExact><<<nb, 256, 0, st>>>(L, O, S, V, C, D);
+  if (C.G == 1 && !fill) {
+    k_count_cover<<<(unsigned)((C.ngp + 255) / 256), 256, 0, st>>>(L, O, S, C, D);
+    return;
+  }
+  const int pass = C.G == 1 ? kFillDiag : (fill ? kFill : kCountExact);
+  switch (L.cls) {
+    case LETKF_LOC_OCEAN: launch_search_cls<LETKF_LOC_OCEAN>(st, pass, nb, L, O, S, V, C, D); break;
+    case LETKF_LOC_VGAUSS: launch_search_cls<LETKF_LOC_VGAUSS>(st, pass, nb, L, O, S, V, C, D); break;
+    default: launch_search_cls<LETKF_LOC_NOVRT>(st, pass, nb, L, O, S, V, C, D); break;
   }
 }
 
