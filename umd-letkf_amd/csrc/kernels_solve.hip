@@ -372,10 +372,7 @@ __global__ __launch_bounds__(256) void k_finalize(LocDev L, StateDev S, VarsDev 
     for (int64_t e0 = 0; e0 < n; e0 += 512) {
       float v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int64_t e = e0 + 64 * u + lane;
-        v[u] = e < n ? xb0[e] : 0.0f;
-      }
+      for (int u = 0; u < 8; ++u) v[u] = xb0[min(e0 + 64 * u + lane, n - 1)];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int64_t e = e0 + 64 * u + lane;

@@ -40,13 +40,14 @@ LK_BS(4) LK_BS(5) LK_BS(6) LK_BS(7) LK_BS(8)
 // same order by every lane (x starts at g_c in row c, is 0 above it), so the
 // pivot reads are broadcasts and the stores of a row are contiguous
 constexpr int kMaxNodes = 32;
+constexpr int kNodeLd = 129;        // LDS row length of the node arrays
 
 // rows rfirst..k-1 of column c of M for the first NN nodes (nn <= NN)
 template <int NN>
 __device__ __forceinline__ void mat_rational_rows(int k, int c, int rfirst, int nn, double m,
                                                   const double* __restrict__ wn,
-                                                  const double (*Us)[128],
-                                                  const double (*Gs)[128], double* Bp) {
+                                                  const double (*Us)[kNodeLd],
+                                                  const double (*Gs)[kNodeLd], double* Bp) {
   double cf[NN], g[NN], x[NN];
 #pragma unroll
   for (int n = 0; n < NN; ++n) {
@@ -80,7 +81,9 @@ __global__ __launch_bounds__(128) void k_mat_rational(int k, ChunkDev C, double*
                                                       const int* __restrict__ tabn,
                                                       int nbucket) {
   __shared__ double sd[128], se[128];
-  __shared__ double Us[kMaxNodes][128], Gs[kMaxNodes][128];
+  // rows of 129: lane n of the pivot recurrences writes row n, and the
+  // lanes' entries then fall in different banks (with 128, all in one)
+  __shared__ double Us[kMaxNodes][kNodeLd], Gs[kMaxNodes][kNodeLd];
   __shared__ double sgm[2];
   const int64_t pair = blockIdx.x;
   if (!C.solve[pair]) return;
@@ -314,24 +317,43 @@ __global__ __launch_bounds__(256) void k_apply_mat(LocDev L, StateDev S, VarsDev
   const bool rows = 32 * wv < k;
   const int i = lane & 31, half = lane >> 5;
   const int ks = (k + 15) / 16;
-  // T rows 32 wv + i, converted once
+  // T rows 32 wv + i, converted once.  The loads are unconditional (a
+  // conditional load waits for each value before the next is issued), 16 at
+  // a time, from one lane offset with constant offsets through a buffer
+  // resource over this problem's T (reads past it return 0); entries past k
+  // are masked after.
   const int row = 32 * wv + i;
-  const double* Tp = B + pair * (int64_t)k * k + (int64_t)row * k;
-  float af[KS][8];
+  const __amdgpu_buffer_rsrc_t trs = buf_rsrc(B + pair * (int64_t)k * k, (int64_t)k * k * 8);
+  const uint32_t tvo = (uint32_t)(min(row, k - 1) * k + 8 * half) * 8u;
+  // FP32 operand (PREC_FP32) or its BF16 split, converted group by group
+  float af[PREC == LETKF_PREC_FP32 ? KS : 1][8];
+  bf16x8 ahi[KS], alo[PREC == LETKF_PREC_BF16X3 ? KS : 1];
 #pragma unroll
-  for (int st = 0; st < KS; ++st)
+  for (int s0 = 0; s0 < KS; s0 += 2) {
+    double tv[2][8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int kk = 16 * st + 8 * half + e;
-      af[st][e] = (rows && st < ks && row < k && kk < k) ? (float)Tp[kk] : 0.0f;
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) tv[u][e] = buf_f64(trs, tvo, 8 * (16 * (s0 + u) + e));
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      float c[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int kk = 16 * (s0 + u) + 8 * half + e;
+        c[e] = (rows && row < k && kk < k) ? (float)tv[u][e] : 0.0f;
+      }
+      if (PREC == LETKF_PREC_FP32) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) af[s0 + u][e] = c[e];
+      } else if (PREC == LETKF_PREC_BF16) {
+        ahi[s0 + u] = to_bf16(c);
+      } else {
+        split_bf16(c, ahi[s0 + u], alo[s0 + u]);
+      }
     }
-  bf16x8 ahi[KS], alo[KS];
-  if (PREC != LETKF_PREC_FP32) {
-#pragma unroll
-    for (int st = 0; st < KS; ++st) {
-      if (PREC == LETKF_PREC_BF16) ahi[st] = to_bf16(af[st]);
-      else split_bf16(af[st], ahi[st], alo[st]);
-    }
+    // one group of loads at a time (16 doubles in registers, not 64)
+    __builtin_amdgcn_sched_barrier(0);
   }
   const float* xb0 = S.state + (int64_t)S.k * S.ns * ij;
   double* xa0 = ana + (int64_t)S.k * S.ns * ij;
@@ -484,24 +506,35 @@ __global__ __launch_bounds__(256) void k_apply_mat_f64(LocDev L, StateDev S, Var
   const float* xb0 = S.state + (int64_t)S.k * S.ns * ij;
   double* xa0 = ana + (int64_t)S.k * S.ns * ij;
   const double* Tb = B + pair * (int64_t)k * k;
+  // every load issued, from one lane offset with constant offsets through
+  // buffer resources over this problem's T and this gridpoint's background
+  // (reads past them return 0); entries past k masked after
+  const __amdgpu_buffer_rsrc_t trs = buf_rsrc(Tb, (int64_t)k * k * 8);
+  const __amdgpu_buffer_rsrc_t xrs = buf_rsrc(xb0, (int64_t)k * S.ns * 4);
   for (int mt = wv; 16 * mt < k; mt += 4) {
     const int row = 16 * mt + i16;
+    const uint32_t tvo = (uint32_t)(min(row, k - 1) * k + kq) * 8u;
     double af[KS];
+#pragma unroll
+    for (int st = 0; st < KS; ++st) af[st] = buf_f64(trs, tvo, 32 * st);
 #pragma unroll
     for (int st = 0; st < KS; ++st) {
       const int kk = 4 * st + kq;
-      af[st] = (st < ks && row < k && kk < k) ? Tb[(int64_t)row * k + kk] : 0.0;
+      af[st] = (row < k && kk < k) ? af[st] : 0.0;
     }
     for (int q0 = 0; q0 < nsl; q0 += 16) {
       const int qi = q0 + i16;
       const int slab = qi < nsl ? sl[qi] : sl[0];
-      const float* xp = xb0 + (int64_t)k * slab;
+      const uint32_t xvo = (uint32_t)(slab * k + kq) * 4u;
+      float xv[KS];
+#pragma unroll
+      for (int st = 0; st < KS; ++st) xv[st] = buf_f32(xrs, xvo, 16 * st);
       f64x4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int st = 0; st < KS; ++st) {
         if (st < ks) {
           const int kk = 4 * st + kq;
-          const double xb = kk < k ? (double)xp[kk] : 0.0;
+          const double xb = kk < k ? (double)xv[st] : 0.0;
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[st], xb, acc, 0, 0, 0);
         }
       }
