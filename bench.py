@@ -88,7 +88,7 @@ def tuning_key(tun):
 def sweep_tunings(spec):
     """Launch choices for --product-sweep: "tuner" is every product-side
     combination the autotuner can reach (order 0..3 with bucket edges 0..2
-    for the bucketed orders, both tile shapes, XCD interleave 0..3), else a
+    for the bucketed orders, both tile shapes, XCD interleave 0..5), else a
     ';'-separated list of order,bucket_bits,xcd,tile."""
     base = dict(tridiag_reg=2, overlap=0, chunk_log2=20)
     out = []
@@ -96,7 +96,7 @@ def sweep_tunings(spec):
         for order in range(4):
             for bb in range(3 if order >= 2 else 1):
                 for tile in (1, 0):
-                    for xcd in range(4):
+                    for xcd in range(6):
                         out.append(dict(order=order, bucket_bits=bb, xcd=xcd, tile=tile, **base))
         return out
     for item in spec.split(";"):

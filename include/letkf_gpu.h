@@ -150,8 +150,9 @@ typedef struct letkf_gpu_timing {
  * gridpoints on a second stream while the current one is solved;
  * chunk_log2: at most 2^chunk_log2 (gridpoint, group) problems per chunk
  * (17..20); xcd: how the product kernel's workgroups meet the 8 XCDs (each
- * with its own L2), 0 in launch order, 1..3 each XCD runs of 32, 64 or 128
- * consecutive problems, the runs of the XCDs interleaved (default 2); tile:
+ * with its own L2), 0 in launch order, 1..5 each XCD runs of 32, 64, 128,
+ * 256 or 512 consecutive problems, the runs of the XCDs interleaved
+ * (default 2); tile:
  * the batched product's MFMA tiles, 0 32x32 blocks over rows padded to a
  * multiple of 32, 1 16x16 tiles over rows padded to a multiple of 16
  * (BF16 and split-BF16 products, k <= 111; 0 elsewhere; the default 1).  None of them

@@ -35,8 +35,8 @@ def test_tuning_keys():
         "order1_bb0_xcd0_c18"
     tun = b.sweep_tunings("tuner")
     keys = [b.tuning_key(t) for t in tun]
-    # order 0 and 1 without bucket edges, 2 and 3 with three each; two tiles; four XCD modes
-    assert len(keys) == len(set(keys)) == (1 + 1 + 3 + 3) * 2 * 4
+    # order 0 and 1 without bucket edges, 2 and 3 with three each; two tiles; six XCD modes
+    assert len(keys) == len(set(keys)) == (1 + 1 + 3 + 3) * 2 * 6
     assert b.product_kernel("batched", "bf16x4", dict(tile=1), 50) == "k_product_t16"
     assert b.product_kernel("batched", "bf16x4", dict(tile=1), 120) == "k_product_mx"
     assert b.product_kernel("batched", "fp32", dict(tile=1), 50) == "k_product_mx"

@@ -301,7 +301,7 @@ int letkf_gpu_create(const letkf_gpu_dims* d, letkf_gpu_t** out) {
   h->ns = d->grid_ns;
   h->nij = d->ij_count;
   if (const char* e = getenv("LETKF_GPU_MEM_LIMIT")) h->mem_limit = strtoull(e, nullptr, 10);
-  if (const char* e = getenv("LETKF_GPU_XCD")) h->tune_xcd = std::min(3, std::max(0, atoi(e)));
+  if (const char* e = getenv("LETKF_GPU_XCD")) h->tune_xcd = std::min(kXcdModes - 1, std::max(0, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_FOLD")) h->fold_log2 = std::min(10, std::max(5, atoi(e)));
   if (const char* e = getenv("LETKF_GPU_TILE")) h->tune_tile = atoi(e) != 0;
   if (const char* e = getenv("LETKF_GPU_FUSE")) h->tune_fuse = atoi(e) != 0;
@@ -1608,7 +1608,7 @@ int letkf_gpu_set_tuning(letkf_gpu_t* h, const letkf_gpu_tuning* t) {
     return fail(h, LETKF_EINVAL, "tuning overlap %d not 0 or 1", t->overlap);
   if (t->chunk_log2 < 17 || t->chunk_log2 > 20)
     return fail(h, LETKF_EINVAL, "tuning chunk_log2 %d outside 17..20", t->chunk_log2);
-  if (t->xcd < 0 || t->xcd > 3)
+  if (t->xcd < 0 || t->xcd >= kXcdModes)
     return fail(h, LETKF_EINVAL, "tuning xcd %d outside 0..3", t->xcd);
   if (t->tile != 0 && t->tile != 1)
     return fail(h, LETKF_EINVAL, "tuning tile %d not 0 or 1", t->tile);
@@ -1684,7 +1684,7 @@ int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out) {
   }
   if (h->batched()) {
     const letkf_gpu_tuning base = best;
-    for (int x = 0; x <= 3; ++x) {
+    for (int x = 0; x < kXcdModes; ++x) {
       if (x == base.xcd) continue;
       letkf_gpu_tuning t = base;
       t.xcd = x;

@@ -91,18 +91,19 @@ struct ChunkDev {
   float* list_w;       // localization weight, or sqrt(weight / err) (list_mode 1)
   int list_mode;       // 0: rloc (FP64 products), 1: sqrt(rloc / err) (mixed)
   int xcd;             // product kernels' block order: 0 as launched, 1..3 runs of
-                       // 32, 64, 128 problems per XCD, interleaved (xcd_order)
+                       // 32 .. 512 problems per XCD, interleaved (xcd_order)
 };
 
 // XCD-interleaved runs: the dispatcher deals workgroups round-robin over
 // the 8 XCDs, each with its own L2.  Remap so that the workgroups of one XCD
-// (b % 8) take runs of R = 16 << mode consecutive problems (mode 1..3: 32,
-// 64, 128) and the runs of the 8 XCDs are interleaved: the problems in
+// (b % 8) take runs of R = 16 << mode consecutive problems (mode 1..5: 32
+// to 512) and the runs of the 8 XCDs are interleaved: the problems in
 // flight on one XCD are neighbours in the launch order (sharing observation
 // columns in that XCD's L2) while the 8 XCDs still move through the order
 // together (a largest-first order stays balanced; giving each XCD one
 // contiguous range would not).  A bijection on [0, nb): the tail past the
 // last whole group of 8 runs keeps its place.  Mode 0: launch order.
+constexpr int kXcdModes = 6;   // xcd_order modes 0..5
 __device__ __forceinline__ int64_t xcd_order(int mode, int64_t b, int64_t nb) {
   constexpr int kXcd = 8;
   if (mode <= 0) return b;
