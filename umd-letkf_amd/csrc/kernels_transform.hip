@@ -42,27 +42,46 @@ LK_BS(4) LK_BS(5) LK_BS(6) LK_BS(7) LK_BS(8)
 constexpr int kMaxNodes = 32;
 constexpr int kNodeLd = 129;        // LDS row length of the node arrays
 
-// rows rfirst..k-1 of column c of M for the first NN nodes (nn <= NN)
+// a wave-uniform double in scalar registers
+__device__ __forceinline__ double uniform_f64(double v) {
+  const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+  const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+// rows rfirst..k-1 of column c of M for the first NN nodes (nn <= NN).  x
+// starts at g_c (the inverse's diagonal) and follows x <- -U_r x from row
+// c + 1 on; rows above c are not stored.  The node weights are uniform
+// (scalar registers), and each row's U values are read 8 at a time before
+// they are used, all lanes reading the same entries.
 template <int NN>
 __device__ __forceinline__ void mat_rational_rows(int k, int c, int rfirst, int nn, double m,
                                                   const double* __restrict__ wn,
                                                   const double (*Us)[kNodeLd],
                                                   const double (*Gs)[kNodeLd], double* Bp) {
-  double cf[NN], g[NN], x[NN];
+  double cf[NN], x[NN];
 #pragma unroll
   for (int n = 0; n < NN; ++n) {
-    cf[n] = n < nn ? m * wn[n] : 0.0;
-    g[n] = Gs[n][c];
-    x[n] = 0.0;
+    cf[n] = uniform_f64(n < nn ? m * wn[n] : 0.0);
+    x[n] = Gs[n][c];
   }
   for (int r = rfirst; r < k; ++r) {
     // four partial sums: the node chains are independent, the sum need not
     // be one dependent chain
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int n = 0; n < NN; ++n) {
-      x[n] = r == c ? g[n] : -Us[n][r] * x[n];
-      acc[n & 3] = fma(cf[n], x[n], acc[n & 3]);
+    for (int n0 = 0; n0 < NN; n0 += 8) {
+      double u[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) u[e] = Us[n0 + e][r];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int n = n0 + e;
+        double xn = -u[e] * x[n];
+        asm volatile("" : "+v"(xn));   // computed on every lane, then selected
+        x[n] = r > c ? xn : x[n];
+        acc[n & 3] = fma(cf[n], x[n], acc[n & 3]);
+      }
     }
     if (r >= c) Bp[(int64_t)r * k + c] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   }

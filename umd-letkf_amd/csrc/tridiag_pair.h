@@ -289,9 +289,12 @@ void k_tridiag_pair(int k, int K, int64_t np, const int32_t* __restrict__ perm, 
       __syncthreads();
       // p = t A v over the columns right of j
       double p0a = 0.0, p0b = 0.0, p1a = 0.0, p1b = 0.0;
+      // each column's v is read one column ahead of its use
+      double vnx = svh[j + 1];
 #pragma unroll
       for (int c = j + 1; c < KQ; ++c) {
-        const double vc = svh[c];
+        const double vc = vnx;
+        if (c + 1 < KQ) vnx = svh[c + 1];
         if (c & 1) {
           if (lo_live) p0b = fma(a0[c], vc, p0b);
           p1b = fma(a1[c], vc, p1b);
@@ -313,9 +316,12 @@ void k_tridiag_pair(int k, int K, int64_t np, const int32_t* __restrict__ perm, 
       svwh[r1].y = w1;
       __syncthreads();
       // rank-2 update A -= v w^T + w v^T on the columns right of j
+      // each column's (v, w) is read one column ahead of its use
+      double2 vwnx = svwh[j + 1];
 #pragma unroll
       for (int c = j + 1; c < KQ; ++c) {
-        const double2 vw = svwh[c];
+        const double2 vw = vwnx;
+        if (c + 1 < KQ) vwnx = svwh[c + 1];
         const double vc = vw.x, wc = vw.y;
         if (lo_live) a0[c] = fma(-w0, vc, fma(-v0, wc, a0[c]));
         a1[c] = fma(-w1, vc, fma(-v1, wc, a1[c]));
