@@ -144,6 +144,23 @@ __device__ __forceinline__ int slab_group(const LocDev& L, const VarsDev& V,
   return 0;
 }
 
+// buffer resource over n bytes at p: loads past the end read 0 (and touch
+// no memory), so a load can be issued for every lane with an offset past
+// the end where its value is not wanted
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr uint32_t kBufOut = 0x7ffffff0u;   // an offset past every buffer
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int64_t n) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)n, 0x00020000);
+}
+// FP64 / FP32 value at byte offset vo + so of a buffer (0 past its end)
+__device__ __forceinline__ double buf_f64(__amdgpu_buffer_rsrc_t r, uint32_t vo, int so) {
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0);
+  return __hiloint2double((int)v[1], (int)v[0]);
+}
+__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, uint32_t vo, int so) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+
 // host launchers (kernels_*.hip)
 void launch_obs_prep(hipStream_t st, int64_t n, int k, const float* lat,
                      const float* lon, int32_t* cell, int32_t* iota);

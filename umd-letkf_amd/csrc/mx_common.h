@@ -11,20 +11,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
-// buffer resource over n bytes at p: loads past the end read 0
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int64_t n) {
-  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)n, 0x00020000);
-}
-// FP64 / FP32 value at byte offset vo + so of a buffer (0 past its end)
-__device__ __forceinline__ double buf_f64(__amdgpu_buffer_rsrc_t r, uint32_t vo, int so) {
-  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0);
-  return __hiloint2double((int)v[1], (int)v[0]);
-}
-__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, uint32_t vo, int so) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
-}
 
 // two values per conversion (v_cvt_pk_bf16_f32, round to nearest even)
 __device__ __forceinline__ bf16x8 to_bf16(const float* v) {
