@@ -586,7 +586,7 @@ def main():
     tun_fixed = None
     if args.tuning:
         v = [int(x) for x in args.tuning.split(",")]
-        v += [2, 0, 20, 2, 1][len(v) - 2:] if len(v) < 7 else []
+        v += [2, 0, 20, 4, 1][len(v) - 2:] if len(v) < 7 else []
         tun_fixed = dict(order=v[0], bucket_bits=v[1], tridiag_reg=v[2], overlap=v[3],
                          chunk_log2=v[4], xcd=v[5], tile=v[6])
     args.tun_fixed = tun_fixed
@@ -625,7 +625,7 @@ def main():
         tuning = solver.tuning()
         log(f"[bench] tuning set: {tuning}")
     elif not args.no_autotune:
-        tuning = solver.autotune(repeats=2)
+        tuning = solver.autotune(repeats=4)
         log(f"[bench] autotune: {tuning}")
     else:
         tuning = solver.tuning()

@@ -142,7 +142,8 @@ typedef struct letkf_gpu_timing {
 /* Launch choices of the analysis (no counterpart in the reference, which
  * has no device kernels to tune).  order: problem order of the batched
  * product, 0 gridpoint order, 1 longest first, 2 buckets of the kept
- * count largest first, 3 buckets with a Morton walk inside; bucket_bits:
+ * count largest first, 3 buckets with a Morton walk inside (the default);
+ * bucket_bits:
  * the bucket edges are the powers of two split into 2^bucket_bits parts
  * (0..2); tridiag_reg: the Householder reduction for 32 <= k < 64, 1 one
  * problem per wave with the matrix in registers, 2 two problems per wave
@@ -152,7 +153,7 @@ typedef struct letkf_gpu_timing {
  * (17..20); xcd: how the product kernel's workgroups meet the 8 XCDs (each
  * with its own L2), 0 in launch order, 1..5 each XCD runs of 32, 64, 128,
  * 256 or 512 consecutive problems, the runs of the XCDs interleaved
- * (default 2); tile:
+ * (default 4); tile:
  * the batched product's MFMA tiles, 0 32x32 blocks over rows padded to a
  * multiple of 32, 1 16x16 tiles over rows padded to a multiple of 16
  * (BF16 and split-BF16 products, k <= 111; 0 elsewhere; the default 1).  None of them

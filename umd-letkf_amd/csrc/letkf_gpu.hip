@@ -142,11 +142,12 @@ struct letkf_gpu_t {
   std::unordered_map<void*, size_t> live;
   size_t live_bytes = 0;
   size_t mem_limit = 0;   // 0 = none
-  int tune_xcd = 2;       // product kernels: runs of 64 problems per XCD
+  int tune_xcd = 4;       // product kernels: runs of 256 problems per XCD
   int tune_tile = 1;      // batched product: 0 32x32 MFMA blocks, 1 16x16 tiles
                           // (where that kernel exists)
   int fold_log2 = 7;      // compensated products: partial sums folded every 2^n obs
-  int tune_bucket = 2;    // 0 gridpoint order, 1 longest first, 2 log2 buckets
+  int tune_bucket = 3;    // 0 gridpoint order, 1 longest first, 2 log2 buckets,
+                          // 3 log2 buckets with a Morton walk inside
   int tune_bbits = 0;     // bucket edges: powers of two split in 2^bbits parts
   int tune_trireg = 2;    // tridiagonalization (32 <= k < 64): 0 LDS, 1 register,
                           // 2 two problems per wave (k <= 50; 1 above)

@@ -217,7 +217,7 @@ class LetkfGpuSolver:
         return {f: getattr(t, f) for f, _ in abi.Tuning._fields_}
 
     def set_tuning(self, order: int, bucket_bits: int = 0, tridiag_reg: int = 2,
-                   overlap: int = 1, chunk_log2: int = 20, xcd: int = 2, tile: int = 1, **_):
+                   overlap: int = 1, chunk_log2: int = 20, xcd: int = 4, tile: int = 1, **_):
         t = abi.Tuning(order, bucket_bits, tridiag_reg, 0, 0.0, 0.0, overlap, chunk_log2, xcd,
                        tile)
         self._chk(self.lib.letkf_gpu_set_tuning(self.h, C.byref(t)))
