@@ -227,15 +227,19 @@ __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// the one-group fill pass (the common case) keeps one group in LDS, which
+// leaves six waves per SIMD to hide the candidates' load latency
 template <int PASS, int CLS>
-__global__ __launch_bounds__(256) void k_search(LocDev L, ObsDev O, StateDev S,
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(PASS == 3 ? 6 : 1, 8))) void k_search(LocDev L, ObsDev O, StateDev S,
                                                 VarsDev V, ChunkDev C, DiagDev D) {
   constexpr bool FILL = PASS == kFill || PASS == kFillDiag;
   constexpr bool STATS = PASS == kCountExact || PASS == kFillDiag;   // sums, hash
-  __shared__ int64_t s_cnt[4][kMaxGroups];
-  __shared__ double s_sum[4][kMaxGroups];
-  __shared__ uint64_t s_hash[4][kMaxGroups];
-  __shared__ int64_t s_lim[4][kMaxGroups];       // fill: end of each list's space
+  constexpr int NG = PASS == kFillDiag ? 1 : kMaxGroups;   // groups held in LDS
+  __shared__ int64_t s_cnt[4][NG];
+  __shared__ double s_sum[4][NG];
+  __shared__ uint64_t s_hash[4][NG];
+  __shared__ int64_t s_lim[4][NG];               // fill: end of each list's space
   __shared__ int32_t s_qs[4][kQueue];             // queued candidates: obs index,
   __shared__ double s_qc[4][kQueue];              //   squared chord,
   __shared__ float s_qe[4][kQueue];               //   error variance
