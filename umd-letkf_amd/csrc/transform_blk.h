@@ -7,9 +7,12 @@
 // transform_mat.h needs v and w for all of its NR rows, broadcast to every
 // thread of the block: the LDS traffic per step falls from about 24 NR to
 // about 4 BS doubles per thread.  The symmetric product's row sums are DPP
-// sums over the 16 threads of a grid row (consecutive lanes, one DPP row).
-// Two barriers per step: after the reflector's column (or v) is in LDS, and
-// after p = t A v and the partial sums of v.p are.
+// sums over the 16 threads of a grid row (consecutive lanes, one DPP row),
+// scattered so that each lane finishes one row.  Two barriers per step:
+// after the reflector's column (or v) is in LDS, and after p = t A v and the
+// partial sums of v.p are.  In the reduction the column goes to LDS with its
+// reduced rows zeroed and the wholly reduced waves keep their p entries at
+// zero, so v = x sc and w = p - kc v need no row tests.
 //
 //  k_blk_tridiag<BS>  A = Q Tri Q^T (LAPACK dsytd2, lower), the outputs of
 //                     k_mat_tridiag: reflector j in row j of B (columns > j),
@@ -44,6 +47,36 @@ __device__ __forceinline__ double row16_sum(double x) {
   x += blk_dpp<0x114>(x);
   x += blk_dpp<0x118>(x);
   return x;
+}
+
+// sums of BS <= 8 values over the 16 lanes of this lane's DPP row, scattered:
+// lane J of the row (J = lane % 16) ends with the whole sum of value J / 2
+// (lanes J and J ^ 1 both hold it; values past BS are zero).  Four exchange
+// stages, each with the lane at distance 8 (row_ror 8), at mirrored position
+// in the half row (row_half_mirror), at distance 2 and 1 (quad_perm): each
+// lane keeps half of the values it carries and sends the other half, so the
+// stages move 4, 2, 1 and 1 values where four shift-and-add stages per value
+// move 4 BS.  The whole wave must be active.
+template <int BS>
+__device__ __forceinline__ double row16_scatter_sums(const double (&s)[BS], int J) {
+  static_assert(BS <= 8, "at most 8 values");
+  const bool h1 = (J & 8) != 0, h2 = (J & 4) != 0, h3 = (J & 2) != 0;
+  double t[4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    const double a = s[x], b = x + 4 < BS ? s[x + 4 < BS ? x + 4 : 0] : 0.0;
+    const double keep = h1 ? b : a, send = h1 ? a : b;
+    t[x] = keep + blk_dpp<0x128>(send);
+  }
+  double u[2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    const double keep = h2 ? t[x + 2] : t[x], send = h2 ? t[x] : t[x + 2];
+    u[x] = keep + blk_dpp<0x141>(send);
+  }
+  const double keep = h3 ? u[1] : u[0], send = h3 ? u[0] : u[1];
+  const double w = keep + blk_dpp<0x4E>(send);
+  return w + blk_dpp<0xB1>(w);
 }
 
 template <int BS>
@@ -93,10 +126,11 @@ __global__ __launch_bounds__(256) void k_blk_tridiag(int k, ChunkDev C, double* 
       const int j = jb * BS + bj;
       if (j >= jend) break;
       const int j1 = j + 1;
-      // column j (local column bj of grid column jb) into LDS
+      // column j (local column bj of grid column jb) into LDS, zero in the
+      // rows already reduced (r <= j)
       if (J == jb) {
 #pragma unroll
-        for (int x = 0; x < BS; ++x) sx[R0 + x] = a[x][bj];
+        for (int x = 0; x < BS; ++x) sx[R0 + x] = R0 + x > j ? a[x][bj] : 0.0;
       }
       __syncthreads();
       // the reflector, the same in every wave
@@ -112,41 +146,49 @@ __global__ __launch_bounds__(256) void k_blk_tridiag(int k, ChunkDev C, double* 
       const double beta = refl ? (alpha >= 0.0 ? -mu : mu) : alpha;
       const double t = refl ? (beta - alpha) / beta : 0.0;
       const double sc = refl ? 1.0 / (alpha - beta) : 0.0;
-      // branch-free: the LDS reads are unconditional and the row tests only
-      // pick factors (a conditional read becomes a branch and a full LDS
-      // wait per element)
+      // v = x sc below row j+1 (the column is zero in rows <= j) and 1 in
+      // row j+1, which is local row (bj + 1) % BS of grid row jb + (bj + 1) / BS:
+      // a register index known at compile time, so one select per thread
+      const int xj1 = (bj + 1) % BS;
+      const int gj1 = jb + (bj + 1) / BS;
       double vr[BS], vc[BS];
 #pragma unroll
       for (int x = 0; x < BS; ++x) {
-        const int r = R0 + x, c = C0 + x;
-        const double xr = sx[r], xc = sx[c];
-        vr[x] = fma(xr, r > j1 ? sc : 0.0, r == j1 ? 1.0 : 0.0);
-        vc[x] = fma(xc, c > j1 ? sc : 0.0, c == j1 ? 1.0 : 0.0);
+        vr[x] = sx[R0 + x] * sc;
+        vc[x] = sx[C0 + x] * sc;
+        if (x == xj1) {   // folded once the step loop is unrolled
+          vr[x] = I == gj1 ? 1.0 : vr[x];
+          vc[x] = J == gj1 ? 1.0 : vc[x];
+        }
       }
       const bool live = wlast > j;   // this wave still has rows below j
       if (live) {
-        // row sums of this block's part of A v, completed over the grid row
+        // row sums of this block's part of A v, completed over the grid row:
+        // lane J holds row R0 + J / 2 (the even lane writes it)
         double P[BS];
 #pragma unroll
         for (int x = 0; x < BS; ++x) {
           double s0 = 0.0;
 #pragma unroll
           for (int y = 0; y < BS; ++y) s0 = fma(a[x][y], vc[y], s0);
-          P[x] = row16_sum(s0);
+          P[x] = s0;
         }
-        double part = 0.0;
+        const double pz = row16_scatter_sums<BS>(P, J);
+        const int xr = J >> 1, r = R0 + xr;
+        const bool own = (J & 1) == 0 && xr < BS;
+        const double px = r > j ? t * pz : 0.0;
+        if (own) sp[r] = px;
+        const double vrow = r == j1 ? 1.0 : sx[min(r, 127)] * sc;
+        const double ws = wave_sum_dpp(own ? px * vrow : 0.0);
+        if (l == 0) skc[wv] = ws;
+      } else {
+        // every row of this wave is reduced: its p entries stay zero, so
+        // that w = p - kc v needs no row test
         if (J == 15) {
 #pragma unroll
-          for (int x = 0; x < BS; ++x) {
-            const double px = R0 + x > j ? t * P[x] : 0.0;
-            sp[R0 + x] = px;
-            part = fma(px, vr[x], part);
-          }
+          for (int x = 0; x < BS; ++x) sp[R0 + x] = 0.0;
         }
-        const double ws = wave_sum_dpp(part);
-        if (l == 0) skc[wv] = ws;
-      } else if (l == 0) {
-        skc[wv] = 0.0;
+        if (l == 0) skc[wv] = 0.0;
       }
       // reflector j (columns > j) and tau, e
       if (I == 0) {
@@ -160,13 +202,12 @@ __global__ __launch_bounds__(256) void k_blk_tridiag(int k, ChunkDev C, double* 
       __syncthreads();
       if (live) {
         const double kc = 0.5 * t * ((skc[0] + skc[1]) + (skc[2] + skc[3]));
+        // p and v are zero in the reduced rows, and so is w
         double wr[BS], wc[BS];
 #pragma unroll
         for (int x = 0; x < BS; ++x) {
-          const int r = R0 + x, c = C0 + x;
-          const double pr = sp[r], pc = sp[c];
-          wr[x] = r > j ? pr - kc * vr[x] : 0.0;
-          wc[x] = c > j ? pc - kc * vc[x] : 0.0;
+          wr[x] = fma(-kc, vr[x], sp[R0 + x]);
+          wc[x] = fma(-kc, vc[x], sp[C0 + x]);
         }
 #pragma unroll
         for (int x = 0; x < BS; ++x)
@@ -261,18 +302,15 @@ __global__ __launch_bounds__(256) void k_blk_back(int k, ChunkDev C, double* __r
       double s0 = 0.0;
 #pragma unroll
       for (int y = 0; y < BS; ++y) s0 = fma(a[x][y], vc[y], s0);
-      P[x] = row16_sum(s0);
+      P[x] = s0;
     }
-    double part = 0.0;
-    if (J == 15) {
-#pragma unroll
-      for (int x = 0; x < BS; ++x) {
-        const double px = t * P[x];
-        sp[R0 + x] = px;
-        part = fma(px, vr[x], part);
-      }
-    }
-    const double ws = wave_sum_dpp(part);
+    // lane J holds row R0 + J / 2 of A v (the even lane writes it)
+    const double pz = row16_scatter_sums<BS>(P, J);
+    const int xr = J >> 1, r = R0 + xr;
+    const bool own = (J & 1) == 0 && xr < BS;
+    const double px = t * pz;
+    if (own) sp[r] = px;
+    const double ws = wave_sum_dpp(own ? px * sv[buf][min(r, 127)] : 0.0);
     if (l == 0) skc[wv] = ws;
     __syncthreads();
     const double kc = 0.5 * t * ((skc[0] + skc[1]) + (skc[2] + skc[3]));
