@@ -190,14 +190,10 @@ __global__ __launch_bounds__(256) void k_blk_tridiag(int k, ChunkDev C, double* 
         }
         if (l == 0) skc[wv] = 0.0;
       }
-      // reflector j (columns > j) and tau, e
-      if (I == 0) {
-#pragma unroll
-        for (int y = 0; y < BS; ++y) {
-          const int c = C0 + y;
-          if (c > j && c < k) Bp[(int64_t)j * k + c] = vc[y];
-        }
-      }
+      // reflector j (columns > j), one entry per thread of the first two
+      // waves (one store each instead of BS predicated stores in the first
+      // grid row), and tau, e
+      if (tid > j && tid < k) Bp[(int64_t)j * k + tid] = tid == j1 ? 1.0 : sx[tid] * sc;
       if (tid == 0) { stau[j] = t; se[j] = beta; }
       __syncthreads();
       if (live) {
