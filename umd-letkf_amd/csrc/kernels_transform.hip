@@ -93,16 +93,24 @@ __device__ __forceinline__ double mat_rcp(double x) {
   const double y = __builtin_amdgcn_rcp(x);
   return y * (2.0 - x * y);
 }
+// One launch per class of node counts NNLO < nn <= NNL: the node arrays are
+// sized for the class, so the problems that need few nodes (the better
+// conditioned ones) run four blocks per CU (16 nodes, 35 KB of LDS) or three
+// (24 nodes) where 32 nodes allow two; a problem of another class leaves once
+// its node count is known (uniform over the block).  Each problem takes the
+// same steps in its class's launch as in a single launch sized for 32 nodes.
+template <int NNL, int NNLO>
 __global__ __launch_bounds__(128) void k_mat_rational(int k, ChunkDev C, double* __restrict__ B,
                                                       const double* __restrict__ dd,
                                                       const double* __restrict__ ee,
                                                       const double* __restrict__ tab,
                                                       const int* __restrict__ tabn,
                                                       int nbucket) {
+  static_assert(NNL % 8 == 0 && NNL <= kMaxNodes, "node class");
   __shared__ double sd[128], se[128];
   // rows of 129: lane n of the pivot recurrences writes row n, and the
   // lanes' entries then fall in different banks (with 128, all in one)
-  __shared__ double Us[kMaxNodes][kNodeLd], Gs[kMaxNodes][kNodeLd];
+  __shared__ double Us[NNL][kNodeLd], Gs[NNL][kNodeLd];
   __shared__ double sgm[2];
   const int64_t pair = blockIdx.x;
   if (!C.solve[pair]) return;
@@ -123,10 +131,11 @@ __global__ __launch_bounds__(128) void k_mat_rational(int k, ChunkDev C, double*
   bk = bk < 1 ? 1 : (bk > nbucket - 1 ? nbucket - 1 : bk);
   bk = __builtin_amdgcn_readfirstlane(bk);
   const int nn = __builtin_amdgcn_readfirstlane(tabn[bk]);
+  if (nn > NNL || nn <= NNLO) return;
   const double* sn = tab + bk * 64;
   const double* wn = sn + 32;
   // unused node rows stay 0, so that their chains stay 0
-  for (int n = nn; n < kMaxNodes; ++n) { Us[n][tid] = 0.0; Gs[n][tid] = 0.0; }
+  for (int n = nn; n < NNL; ++n) { Us[n][tid] = 0.0; Gs[n][tid] = 0.0; }
   // one lane per node runs both pivot recurrences in one loop, step s
   // taking the top-down pivot D+ of row s and the bottom-up pivot D- of row
   // k-1-s (two independent chains).  U_i = e_(i-1) / D-_i goes to Us; the
@@ -165,9 +174,7 @@ __global__ __launch_bounds__(128) void k_mat_rational(int k, ChunkDev C, double*
   // rows above a wave's first column hold zeros in every lane of the wave
   const int rfirst = __builtin_amdgcn_readfirstlane(tid & ~63);
   double* Bp = B + pair * (int64_t)k * k;
-  if (nn <= 16) mat_rational_rows<16>(k, c, rfirst, nn, m, wn, Us, Gs, Bp);
-  else if (nn <= 24) mat_rational_rows<24>(k, c, rfirst, nn, m, wn, Us, Gs, Bp);
-  else mat_rational_rows<kMaxNodes>(k, c, rfirst, nn, m, wn, Us, Gs, Bp);
+  mat_rational_rows<NNL>(k, c, rfirst, nn, m, wn, Us, Gs, Bp);
 }
 
 // sum over a slab of the tile: the lane's registers (summed by the caller),
@@ -598,7 +605,9 @@ void launch_transform_mat(hipStream_t st, int prec, const LocDev& L, const State
   else if (nr <= 48) launch_mat_tridiag_nr48(st, nb, k, C, B, tau, dd, ee);
   else if (nr <= 56) launch_mat_tridiag_nr56(st, nb, k, C, B, tau, dd, ee);
   else launch_mat_tridiag_nr64(st, nb, k, C, B, tau, dd, ee);
-  k_mat_rational<<<nb, 128, 0, st>>>(k, C, B, dd, ee, R.tab, R.n, R.nb);
+  k_mat_rational<16, 0><<<nb, 128, 0, st>>>(k, C, B, dd, ee, R.tab, R.n, R.nb);
+  k_mat_rational<24, 16><<<nb, 128, 0, st>>>(k, C, B, dd, ee, R.tab, R.n, R.nb);
+  k_mat_rational<kMaxNodes, 24><<<nb, 128, 0, st>>>(k, C, B, dd, ee, R.tab, R.n, R.nb);
   if (blk) {
     switch (bs) {
       case 4: launch_blk_back_bs4(st, nb, k, C, B, bvec, tau); break;
