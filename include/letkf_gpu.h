@@ -326,8 +326,8 @@ int letkf_gpu_get_timing(const letkf_gpu_t* h, letkf_gpu_timing* t);
 
 /* Autotuner (no counterpart in the reference): times the analysis of the
  * handle's current inputs under each launch choice (coordinate search:
- * order and bucket edges, the tridiagonalization kernel, the chunk overlap,
- * then the chunk size), `repeats`
+ * order and bucket edges, the tridiagonalization kernel, then the chunk
+ * size with the chunk overlap on and off), `repeats`
  * runs each (best taken, 1..16), keeps the fastest (a candidate must beat
  * the current choice by 0.5%) in the handle and
  * returns it.  Needs everything letkf_gpu_run needs; leaves the analysis

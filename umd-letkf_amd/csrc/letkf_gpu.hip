@@ -1702,18 +1702,18 @@ int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out) {
     }
   }
   // overlap and chunk size matter only when the smallest chunk size would
-  // split the analysis
+  // split the analysis; both overlap settings for every chunk size (the
+  // overlap only pays with several chunks, so it is not judged alone)
   if (h->nij * h->G > (1ll << 17)) {
-    letkf_gpu_tuning t = best;
-    t.overlap = 1 - t.overlap;
-    if ((rc = consider(t))) return rc;
     const letkf_gpu_tuning base = best;
-    for (int cl = 17; cl <= 20; ++cl) {
-      if (cl == base.chunk_log2) continue;
-      letkf_gpu_tuning u = base;
-      u.chunk_log2 = cl;
-      if ((rc = consider(u))) return rc;
-    }
+    for (int cl = 17; cl <= 20; ++cl)
+      for (int ov = 0; ov <= 1; ++ov) {
+        if (cl == base.chunk_log2 && ov == base.overlap) continue;
+        letkf_gpu_tuning u = base;
+        u.chunk_log2 = cl;
+        u.overlap = ov;
+        if ((rc = consider(u))) return rc;
+      }
   }
   // leave the chosen settings and their analysis in the handle
   rc = letkf_gpu_set_tuning(h, &best);
