@@ -237,9 +237,12 @@ __global__ __launch_bounds__(256) void k_blk_back(int k, ChunkDev C, double* __r
                                                   const double* __restrict__ bvec,
                                                   const double* __restrict__ tau) {
   static_assert(16 * BS <= 128, "at most 128 rows");
-  __shared__ double sv[2][128];     // reflector of the step (double buffered)
-  __shared__ double sp[128];        // p = t A v; then W b, W (W b)
-  __shared__ double skc[4];
+  // one barrier per step: the reflectors, p and the wave sums are double
+  // buffered by step, and the next step's reflector goes to LDS during this
+  // step's product
+  __shared__ double sv[2][128];     // reflectors of this step and the next
+  __shared__ double sp[2][128];     // p = t A v; then W b, W (W b)
+  __shared__ double skc[2][4];
   __shared__ double stau[128];
   const int64_t pair = blockIdx.x;
   if (!C.solve[pair]) return;
@@ -265,26 +268,22 @@ __global__ __launch_bounds__(256) void k_blk_back(int k, ChunkDev C, double* __r
       if (R0 + x >= k || C0 + y >= k) a[x][y] = 0.0;
   if (tid < 128) {
     stau[tid] = tid < k - 2 ? tau[pair * kBlkStride + tid] : 0.0;
-    sv[0][tid] = 0.0;
-    sv[1][tid] = 0.0;
-    sp[tid] = 0.0;
+    sp[0][tid] = 0.0;
+    sp[1][tid] = 0.0;
   }
-  // reflector rows go through LDS: threads 0..127 hold entry tid of the next
-  // one (loaded a step ahead)
+  // reflector rows go through LDS: reflector k-3 now, and threads 0..127
+  // hold entry tid of the one after next (loaded two steps ahead)
   double vnext = 0.0;
-  if (k - 3 >= 0 && tid < 128) {
+  if (tid < 128) {
     const int j = k - 3;
-    vnext = (tid > j && tid < k) ? Bp[(int64_t)j * k + tid] : 0.0;
+    sv[j & 1][tid] = (tid > j && tid < k) ? Bp[(int64_t)j * k + tid] : 0.0;
+    sv[(j + 1) & 1][tid] = 0.0;
+    const int jn = j - 1;
+    if (jn >= 0) vnext = (tid > jn && tid < k) ? Bp[(int64_t)jn * k + tid] : 0.0;
   }
   __syncthreads();
   for (int j = k - 3; j >= 0; --j) {
     const int buf = j & 1;
-    if (tid < 128) sv[buf][tid] = vnext;
-    if (tid < 128 && j > 0) {
-      const int jn = j - 1;
-      vnext = (tid > jn && tid < k) ? Bp[(int64_t)jn * k + tid] : 0.0;
-    }
-    __syncthreads();
     const double t = stau[j];
     double vr[BS], vc[BS];
 #pragma unroll
@@ -305,16 +304,23 @@ __global__ __launch_bounds__(256) void k_blk_back(int k, ChunkDev C, double* __r
     const int xr = J >> 1, r = R0 + xr;
     const bool own = (J & 1) == 0 && xr < BS;
     const double px = t * pz;
-    if (own) sp[r] = px;
+    if (own) sp[buf][r] = px;
     const double ws = wave_sum_dpp(own ? px * sv[buf][min(r, 127)] : 0.0);
-    if (l == 0) skc[wv] = ws;
+    if (l == 0) skc[buf][wv] = ws;
+    // reflector j-1 into the other buffer (its last reader, step j+1, has
+    // passed the barrier of step j+1); reflector j-2 loaded
+    if (tid < 128 && j > 0) {
+      sv[buf ^ 1][tid] = vnext;
+      const int jn = j - 2;
+      if (jn >= 0) vnext = (tid > jn && tid < k) ? Bp[(int64_t)jn * k + tid] : 0.0;
+    }
     __syncthreads();
-    const double kc = 0.5 * t * ((skc[0] + skc[1]) + (skc[2] + skc[3]));
+    const double kc = 0.5 * t * ((skc[buf][0] + skc[buf][1]) + (skc[buf][2] + skc[buf][3]));
     double wr[BS], wc[BS];
 #pragma unroll
     for (int x = 0; x < BS; ++x) {
-      wr[x] = sp[R0 + x] - kc * vr[x];
-      wc[x] = sp[C0 + x] - kc * vc[x];
+      wr[x] = sp[buf][R0 + x] - kc * vr[x];
+      wc[x] = sp[buf][C0 + x] - kc * vc[x];
     }
 #pragma unroll
     for (int x = 0; x < BS; ++x)
@@ -338,7 +344,7 @@ __global__ __launch_bounds__(256) void k_blk_back(int k, ChunkDev C, double* __r
       s0 = row16_sum(s0);
       if (J == 15) {
         if (pass == 0) sv[1][R0 + x] = s0;
-        else sp[R0 + x] = s0 / m;
+        else sp[0][R0 + x] = s0 / m;
       }
     }
     __syncthreads();
@@ -348,7 +354,7 @@ __global__ __launch_bounds__(256) void k_blk_back(int k, ChunkDev C, double* __r
 #pragma unroll
     for (int y = 0; y < BS; ++y) {
       const int r = R0 + x, c = C0 + y;
-      if (r < k && c < k) Bp[(int64_t)r * k + c] = a[x][y] + sp[c];
+      if (r < k && c < k) Bp[(int64_t)r * k + c] = a[x][y] + sp[0][c];
     }
 }
 
