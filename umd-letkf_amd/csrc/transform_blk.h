@@ -8,9 +8,11 @@
 // thread of the block: the LDS traffic per step falls from about 24 NR to
 // about 4 BS doubles per thread.  The symmetric product's row sums are DPP
 // sums over the 16 threads of a grid row (consecutive lanes, one DPP row),
-// scattered so that each lane finishes one row.  Two barriers per step:
-// after the reflector's column (or v) is in LDS, and after p = t A v and the
-// partial sums of v.p are.  In the reduction the column goes to LDS with its
+// scattered so that each lane finishes one row.  The reduction takes two
+// barriers per step: after the column is in LDS, and after p = t A v and the
+// partial sums of v.p are; the back-transform, whose reflectors are known in
+// advance, only the second (its LDS arrays double buffered by step).  In the
+// reduction the column goes to LDS with its
 // reduced rows zeroed and the wholly reduced waves keep their p entries at
 // zero, so v = x sc and w = p - kc v need no row tests.
 //
