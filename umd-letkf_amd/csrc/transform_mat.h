@@ -22,6 +22,8 @@
 //                     from the LDL^T pivots of Tri + s I: x_c = 1/g_c with
 //                     g_c = D+_c + D-_c - (d_c + s), x_i = -(e_(i-1) / D-_i)
 //                     x_(i-1) for i > c (D+ top-down, D- bottom-up pivots).
+//                     Launched once per class of node counts (<= 16, 24,
+//                     32), its LDS sized for the class.
 //  k_mat_back<NR>     the register layout of k_mat_tridiag: W = Q M Q^T by
 //                     the reflectors applied from both sides in reverse
 //                     order (the same two-sided rank-2 update), then
