@@ -115,3 +115,18 @@ def test_block_and_column_kernels_agree(prec, path, k, monkeypatch):
         for out in (blk, col):
             e = dict(mean=rel(out["mean"], orc["mean"]), sprd=rel(out["sprd"], orc["sprd"]))
             assert max(e.values()) <= TOL[prec], e
+
+
+@pytest.mark.parametrize("err", [0.05, 3e-3, 3e-4])
+def test_conditioning_classes(err):
+    """The rational M runs once per class of node counts (at most 16, 24 and
+    32 nodes, picked per problem from the Gershgorin bound of Tri; about
+    log2(kappa) + 4 nodes): smaller observation errors raise the condition
+    number kappa of (k-1) I + Y^T R^-1 Y (from the largest eigenvalue over
+    the oracle's kept sets of every 97th gridpoint: 114..291 at err = 0.05,
+    3.2e4..8.1e4 at 3e-3, 3.2e6..8.1e6 at 3e-4), so the three cases need at
+    least 11..13, 19..21 and 26..27 nodes and put their problems in the three
+    classes.  The plain FP64 path meets the oracle to 1e-10 in each."""
+    p = synth.random_problem(seed=1300, k=100, ns=3, nobs=2500, err=err, rtps=0.0)
+    print("conditioning err", err, flush=True)
+    compare(p, run_gpu(p, precision="fp64", path="plain"), oracle_run(p), tol=1e-10)
