@@ -1703,7 +1703,7 @@ int letkf_gpu_autotune(letkf_gpu_t* h, int32_t repeats, letkf_gpu_tuning* out) {
   }
   // overlap and chunk size matter only when the smallest chunk size would
   // split the analysis; both overlap settings for every chunk size (the
-  // overlap only pays with several chunks, so it is not judged alone)
+  // overlap only pays with several chunks, so it is not tried alone)
   if (h->nij * h->G > (1ll << 17)) {
     const letkf_gpu_tuning base = best;
     for (int cl = 17; cl <= 20; ++cl)
